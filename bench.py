@@ -1,0 +1,156 @@
+"""Benchmark: the single-column WAH index build at 2^28 uint32 values.
+
+Workload: 2^28 values i.i.d. uniform in [0, 65535], seed 1 (the single-column
+GPU drop-in configuration), resident in HBM before timing starts.  A step is
+one fused build (wahgpu_build_device: histogram, radix passes, tail, lengths)
+over the whole column.  W warm-up steps, then K timed steps between
+synchronizations; value = n * K / elapsed in Gvalues/s.
+
+Beside it:
+  roofline      the dominant kernel (the radix passes, k_onesweep) timed with
+                HIP events on the build's stream over the timed steps;
+                achieved = its algorithmic bytes per launch / its mean launch
+                time (pass 1: 4n read + 8n written; later passes 16n)
+  build_roofline the same for the whole build: B_alg = 16nP + 8n + 4L + 12K
+  cpu_baseline  the oracle (a C restatement of the reference's serial `cpu`
+                program, one thread) on a 2^26-value sample of the same
+                distribution, timed on this host
+Prints one JSON line.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "indexed Gvalues/s + HBM-roofline % (2^28 u32); p50/p99 serve latency"
+HBM_PEAK_GBS = 8000.0
+N_VALUES = 1 << 28
+CPU_SAMPLE = 1 << 26
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "traffic_onesweep_uniform_2p28.txt")
+
+
+def read_traffic():
+    """HBM bytes per k_onesweep launch measured with rocprofv3 counters
+    (FETCH_SIZE + WRITE_SIZE, calibrated on a copy kernel), if recorded."""
+    try:
+        for line in open(TRAFFIC_FILE):
+            if line.startswith("bytes_per_launch:"):
+                return float(line.split(":", 1)[1])
+    except OSError:
+        pass
+    return None
+
+
+def cpu_baseline(sample_n):
+    import numpy as np
+
+    import indexing_amd as ia
+    from oracle import oracle
+    values = ia.generate(sample_n, "uniform", seed=1, lo=0, hi=65535)
+    t0 = time.perf_counter()
+    r = oracle.build(values)
+    dt = time.perf_counter() - t0
+    assert r["keycnt"] == np.unique(values).size
+    return {"value": sample_n / dt / 1e9, "unit": "Gvalues/s", "cores": 1, "kind": "port",
+            "sample": f"first 2^{sample_n.bit_length() - 1} values of the same column "
+                      f"(uniform [0,65535], seed 1), oracle build timed single-threaded, "
+                      f"{dt:.2f} s"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--n", type=int, default=N_VALUES, help="column length (default 2^28)")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
+              "(the multi-GPU sharding is a design note only)", file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+
+    import indexing_amd as ia
+
+    n = args.n
+    host = ia.generate(n, "uniform", seed=1, lo=0, hi=65535)
+    d_values = torch.from_numpy(host.view(np.int32)).to("cuda:0")
+    ctx = ia.Context(0)
+    out = ctx.build_device(d_values)                   # allocates outputs + arena
+    for _ in range(args.warmup):
+        ctx.build_device(d_values, out=out)
+    torch.cuda.synchronize()
+
+    ctx.timing_begin(args.steps)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        ctx.build_device(d_values, out=out)
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    group_ms, calls = ctx.timing_end()
+    counts = ctx.read_counts(out[4])
+    assert counts.fault == 0, "device reported a fault"
+    assert calls == args.steps
+
+    ms_per_step = elapsed * 1e3 / args.steps
+    value = n * args.steps / elapsed / 1e9
+
+    # sizes for the byte model
+    L, K, P = int(counts.length), int(counts.keycnt), int(counts.passes)
+    pass_ms = [group_ms[1 + p] / calls for p in range(4)]
+    active = [p for p in range(4) if pass_ms[p] > 0.02]   # inactive passes exit at once
+    assert len(active) == P, (pass_ms, P)
+    pass_bytes = [12.0 * n] + [16.0 * n] * (P - 1)         # per active launch, in order
+    sort_ms = sum(pass_ms[p] for p in active)
+    per_launch_bytes = sum(pass_bytes) / P
+    per_launch_ms = sort_ms / P
+    achieved = per_launch_bytes / (per_launch_ms * 1e-3) / 1e9
+    traffic = read_traffic()
+    b_alg = 16.0 * n * P + 8.0 * n + 4.0 * L + 12.0 * K
+    build_ms_events = sum(group_ms) / calls
+
+    result = {
+        "metric": METRIC,
+        "value": round(value, 3),
+        "unit": "Gvalues/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "u32",
+        "data": "synthetic (seeded splitmix64 uniform column, generated on the host)",
+        "config": {"workload": f"single-column WAH build, n=2^{n.bit_length() - 1} uint32 "
+                               "uniform [0,65535], seed 1, resident in HBM",
+                   "n": n, "k": int(counts.k), "L": L, "K": K, "radix_passes": P},
+        "roofline": {"bound": "hbm", "kernel": "k_onesweep (radix pass)",
+                     "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 4),
+                     "traffic": traffic,
+                     "alg_bytes_per_launch": per_launch_bytes,
+                     "mean_launch_ms": round(per_launch_ms, 4)},
+        "build_roofline": {"alg_bytes": b_alg, "ms": round(build_ms_events, 4),
+                           "achieved": round(b_alg / (build_ms_events * 1e-3) / 1e9, 1),
+                           "frac": round(b_alg / (build_ms_events * 1e-3) / 1e9 / HBM_PEAK_GBS,
+                                         4)},
+        "kernel_ms": {name: round(g / calls, 4) for name, g in zip(
+            ["histogram", "pass0", "pass1", "pass2", "pass3", "tail", "lengths"], group_ms)},
+    }
+    if not args.no_cpu_baseline:
+        result["cpu_baseline"] = cpu_baseline(min(CPU_SAMPLE, n))
+    ctx.close()
+    print(json.dumps(result))
+
+
+if __name__ == "__main__":
+    main()

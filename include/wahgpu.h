@@ -1,0 +1,234 @@
+/*
+ * wahgpu.h -- C ABI of libwahgpu, the MI355X (gfx950) WAH bitmap-index build.
+ *
+ * This header is the drop-in boundary that replaces the CAF OpenCL module
+ * (caf::opencl::manager::spawn + mem_ref<T>) of josephnoir/indexing for the
+ * index-build path.  Only plain C types cross it: pointers, sizes, status
+ * codes.  Every function returns WAHGPU_OK (0) or a negative WAHGPU_E* code;
+ * wahgpu_last_error() gives the message of the last failure on a context.
+ *
+ * Semantics follow the reference's serial program src/cpu.cpp bit for bit:
+ *   rid = row position; chid = rid / 31; literal = 0x80000000 | 1 << rid % 31;
+ *   literals sharing (value, chid) are OR-merged; a key's first fill is its
+ *   first chid, later fills are chid - chid_prev - 1; zero fills are dropped;
+ *   keys ascend; offsets are the exclusive scan of per-key word counts.
+ *
+ * Pointers named d_* are device pointers (hipMalloc / torch CUDA tensors);
+ * the others are host pointers.  Device-pointer functions are ordered on the
+ * context's stream (wahgpu_stream) and return before the GPU finishes unless
+ * they return counts to the host, which they say.
+ *
+ * Limits: one column holds at most WAHGPU_MAX_VALUES values (digit counts and
+ * word positions are kept in 30-bit look-back fields); rid_base + n must not
+ * exceed 2^32 (rows are uint32 as in the reference).
+ */
+#ifndef WAHGPU_H
+#define WAHGPU_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define WAHGPU_OK 0
+#define WAHGPU_EINVAL (-1)   /* bad argument (null pointer, capacity too small) */
+#define WAHGPU_ENOMEM (-2)   /* device arena or host allocation failed          */
+#define WAHGPU_EHIP (-3)     /* HIP runtime error                               */
+#define WAHGPU_ERANGE (-4)   /* n above WAHGPU_MAX_VALUES or rids overflow      */
+#define WAHGPU_ENODEV (-5)   /* no such device                                  */
+#define WAHGPU_EIO (-6)      /* file could not be read or written               */
+#define WAHGPU_EBUSY (-7)    /* serving queue full / request not finished       */
+
+#define WAHGPU_MAX_VALUES ((((uint64_t)1) << 30) - 1u)
+
+typedef struct wahgpu_ctx wahgpu_ctx;
+
+/* Sizes the build produced.  The reference prints k ("Merged to k values",
+ * src/cpu.cpp:321) and L, K ("Index has L elements with K keys", :327). */
+typedef struct wahgpu_counts {
+  uint64_t k;       /* merged (value, chunk) entries                 */
+  uint64_t length;  /* index_length L: WAH words                     */
+  uint64_t keycnt;  /* K: distinct values                            */
+  uint32_t bound;   /* maximum value (0 for an empty column)          */
+  uint32_t passes;  /* radix passes the sort ran (trivial digits skipped) */
+  uint32_t fault;   /* nonzero: a kernel gave up waiting on a look-back
+                       status or caught an out-of-range index; the
+                       outputs are not valid (never expected)            */
+  uint32_t reserved;
+} wahgpu_counts;
+
+/* Microseconds per phase, the buckets `phases` prints (src/phases.cpp:714-1127). */
+typedef struct wahgpu_phase_times {
+  double transfer_to, sort, chunks, merge, fills, fuse, column_length,
+      transfer_back, total;
+} wahgpu_phase_times;
+
+/* ---- devices and contexts (caf::opencl::manager, src/phases.cpp:373-388) */
+int wahgpu_device_count(int* count);
+/* Writes the device's marketing name ("AMD Instinct MI355X") into buf. */
+int wahgpu_device_name(int ordinal, char* buf, size_t cap);
+/* Compute units of the device (the `-j 0` default of src/gpu.cpp:341). */
+int wahgpu_device_compute_units(int ordinal);
+/* Ordinal of the first device whose name or gfx arch equals `name`, or -1
+ * (find_device_if with a name match, src/phases.cpp:376-380). */
+int wahgpu_find_device(const char* name);
+/* One HIP stream plus a pooled device arena.  arena_bytes = 0 grows the arena
+ * on demand; otherwise the arena is allocated once and requests beyond it
+ * fail with WAHGPU_ENOMEM.  A context is used by one thread at a time. */
+int wahgpu_create(wahgpu_ctx** out, int device_ordinal, uint64_t arena_bytes);
+void wahgpu_destroy(wahgpu_ctx* ctx);
+const char* wahgpu_last_error(const wahgpu_ctx* ctx);
+void* wahgpu_stream(wahgpu_ctx* ctx);               /* the hipStream_t */
+/* Orders the context's work on a caller's hipStream_t (e.g. the framework's
+ * current stream); NULL returns to the context's own stream. */
+int wahgpu_set_stream(wahgpu_ctx* ctx, void* stream);
+int wahgpu_synchronize(wahgpu_ctx* ctx);
+uint64_t wahgpu_arena_bytes(const wahgpu_ctx* ctx); /* currently reserved */
+
+/* ---- whole build (the indexer's worker, src/gpu.cpp:356-367; the phases
+ * pipeline, src/phases.cpp:720-1088) ---------------------------------------
+ * Fused fast path on device pointers.  Caller capacities follow the
+ * reference's worst-case sizing (src/gpu.cpp:347-352): d_keys, d_offsets,
+ * d_lengths hold n entries, d_words 2n.  Counts are written to the device
+ * struct d_counts; nothing is returned to the host (graph-capturable). */
+int wahgpu_build_device(wahgpu_ctx* ctx, const uint32_t* d_values, uint64_t n,
+                        uint32_t rid_base, uint32_t* d_keys, uint32_t* d_offsets,
+                        uint32_t* d_lengths, uint32_t* d_words,
+                        wahgpu_counts* d_counts);
+
+/* Per-kernel timing of wahgpu_build_device calls, for measurement: between
+ * wahgpu_timing_begin and wahgpu_timing_end every build_device call records
+ * HIP events around its kernels on the context's stream (at most max_calls
+ * calls).  timing_end waits for them and writes, per kernel group, the summed
+ * milliseconds over the recorded calls:
+ *   ms[0] histogram + digit scan, ms[1..4] radix passes of digits 0..3
+ *   (inactive passes are near zero), ms[5] tail, ms[6] lengths.
+ * *calls receives the number of recorded calls. */
+#define WAHGPU_TIMING_GROUPS 7
+int wahgpu_timing_begin(wahgpu_ctx* ctx, uint32_t max_calls);
+int wahgpu_timing_end(wahgpu_ctx* ctx, double* ms, uint32_t* calls);
+
+/* Host-pointer build: copies values in, runs the fused build, copies the
+ * results out and returns the counts.  keys_cap bounds keys/offsets/lengths,
+ * words_cap bounds words (n and 2n always suffice).  times may be NULL. */
+int wahgpu_build(wahgpu_ctx* ctx, const uint32_t* values, uint64_t n,
+                 uint32_t rid_base, uint32_t* keys, uint32_t* offsets,
+                 uint32_t* lengths, uint64_t keys_cap, uint32_t* words,
+                 uint64_t words_cap, wahgpu_counts* counts,
+                 wahgpu_phase_times* times);
+
+/* Host-pointer build through the staged kernels, one stage per phase, the
+ * way `phases` runs; fills every bucket of times (may be NULL). */
+int wahgpu_build_staged(wahgpu_ctx* ctx, const uint32_t* values, uint64_t n,
+                        uint32_t* keys, uint32_t* offsets, uint32_t* lengths,
+                        uint64_t keys_cap, uint32_t* words, uint64_t words_cap,
+                        wahgpu_counts* counts, wahgpu_phase_times* times);
+
+/* ---- stages (device pointers; each synchronizes when it returns a count) */
+/* include/sort_rids_by_value.cl:24-30 + include/radix.cl:65-233: stable sort
+ * of (value, rid = rid_base + i) by value. */
+int wahgpu_sort_rids_by_value(wahgpu_ctx* ctx, const uint32_t* d_values, uint64_t n,
+                              uint32_t rid_base, uint32_t* d_keys_out,
+                              uint32_t* d_rids_out);
+/* include/produce_chunk_id_literals.cl:17-24 */
+int wahgpu_produce_chunk_id_literals(wahgpu_ctx* ctx, const uint32_t* d_rids,
+                                     uint64_t n, uint32_t* d_chids, uint32_t* d_lits);
+/* include/merge_lit_by_val_chids.cl:32-66 + compaction (src/phases.cpp:864-903):
+ * OR of lits over runs of equal (value, chid); returns k. */
+int wahgpu_merge_lit_by_val_chids(wahgpu_ctx* ctx, const uint32_t* d_values,
+                                  const uint32_t* d_chids, const uint32_t* d_lits,
+                                  uint64_t n, uint32_t* d_values_out,
+                                  uint32_t* d_chids_out, uint32_t* d_lits_out,
+                                  uint64_t* k);
+/* include/produce_fills.cl:24-43 */
+int wahgpu_produce_fills(wahgpu_ctx* ctx, const uint32_t* d_values,
+                         const uint32_t* d_chids, uint64_t k, uint32_t* d_fills);
+/* include/fuse_fill_literals.cl:25-36 + compaction (src/phases.cpp:958-979):
+ * interleave (fill, literal), drop zero words; d_index holds 2k; returns L. */
+int wahgpu_fuse_fill_literals(wahgpu_ctx* ctx, const uint32_t* d_fills,
+                              const uint32_t* d_lits, uint64_t k, uint32_t* d_index,
+                              uint64_t* index_length);
+/* include/compute_column_length.cl:34-70 + segmented scan + compaction + scan
+ * (src/phases.cpp:989-1077): keys, per-key word counts, offsets; returns K. */
+int wahgpu_compute_column_length(wahgpu_ctx* ctx, const uint32_t* d_values,
+                                 const uint32_t* d_fills, uint64_t k,
+                                 uint32_t* d_keys, uint32_t* d_offsets,
+                                 uint32_t* d_lengths, uint64_t* keycnt);
+
+/* ---- primitives -------------------------------------------------------- */
+/* include/stream_compaction.cl:68-160: keep d_data[i] where d_valid[i] != 0,
+ * in order; returns the count (the kernel's result[0]). */
+int wahgpu_stream_compaction(wahgpu_ctx* ctx, const uint32_t* d_data,
+                             const uint32_t* d_valid, uint64_t n, uint32_t* d_out,
+                             uint64_t* count);
+/* include/scan.cl:16-130: exclusive sum scan (uint32, wraps mod 2^32). */
+int wahgpu_exclusive_scan(wahgpu_ctx* ctx, const uint32_t* d_in, uint64_t n,
+                          uint32_t* d_out);
+/* include/segmented_scan.cl:23-271: inclusive sum scan restarting where
+ * d_heads[i] != 0 (element 0 always starts a segment). */
+int wahgpu_segmented_inclusive_scan(wahgpu_ctx* ctx, const uint32_t* d_in,
+                                    const uint32_t* d_heads, uint64_t n,
+                                    uint32_t* d_out);
+
+/* ---- stitching partial indexes (the merger actor, src/gpu.cpp:103-165) ---
+ * Partial indexes built over consecutive row ranges (wahgpu_build with
+ * rid_base = first row of the range) are appended in row order; each key's
+ * words are concatenated with its first fill re-based on the key's previous
+ * chunk, and a chunk shared across the boundary is OR-merged.  The result is
+ * the index of the whole column.  The reference merger (src/gpu.cpp:140-149)
+ * concatenates without re-basing; this one is exact.  Host only. */
+typedef struct wahgpu_merger wahgpu_merger;
+int wahgpu_merger_create(wahgpu_merger** out);
+void wahgpu_merger_destroy(wahgpu_merger* m);
+/* The `add` message: keycnt keys, offsets into words, `length` words total
+ * (per-key lengths are implied, src/gpu.cpp:143-144). */
+int wahgpu_merger_add(wahgpu_merger* m, uint64_t keycnt, uint64_t length,
+                      const uint32_t* keys, const uint32_t* offsets,
+                      const uint32_t* words);
+uint64_t wahgpu_merger_keycnt(const wahgpu_merger* m);
+uint64_t wahgpu_merger_length(const wahgpu_merger* m);
+/* Writes the merged index: keys ascending, offsets, lengths, words. */
+int wahgpu_merger_export(const wahgpu_merger* m, uint32_t* keys, uint32_t* offsets,
+                         uint32_t* lengths, uint32_t* words);
+/* Borrowed view of one key's words (NULL, 0 for an absent key). */
+int wahgpu_merger_key_words(const wahgpu_merger* m, uint32_t key,
+                            const uint32_t** words, uint64_t* len);
+
+/* ---- host-side formats (no GPU needed) --------------------------------- */
+/* Value files as the reference reads them (src/cpu.cpp:297-303): whitespace-
+ * separated decimal, parsed like `ifstream >> uint32_t`, stopping silently at
+ * the first failed extraction.  *values is malloc'ed; free with wahgpu_free. */
+int wahgpu_read_values_text(const char* path, uint32_t** values, uint64_t* n);
+/* Raw little-endian uint32 files (opt-in extension). */
+int wahgpu_read_values_binary(const char* path, uint32_t** values, uint64_t* n);
+void wahgpu_free(void* p);
+/* src/cpu.cpp:47-74: '0'/'1' text of a key's bitmap, trailing '0's trimmed.
+ * Writes at most cap bytes (no terminator); returns the full length. */
+uint64_t wahgpu_decode_bitmap(const uint32_t* words, uint64_t len, char* out,
+                              uint64_t cap);
+
+/* ---- synthetic columns (the `generate` program, src/generate_test_data.cpp)
+ * Seeded and reproducible on any host: value i is a pure function of
+ * (seed, i) for the uniform kind. */
+#define WAHGPU_DIST_UNIFORM 0   /* i.i.d. uniform in [from, to]                  */
+#define WAHGPU_DIST_ZIPF 1      /* Zipf(s) rank - 1 over `cardinality` values     */
+#define WAHGPU_DIST_ZIPF_RUNS 2 /* Zipf(s) values in runs of Geometric(1/run_mean) */
+typedef struct wahgpu_gen_params {
+  uint32_t dist;
+  uint32_t from, to;        /* uniform range (inclusive)                      */
+  uint32_t cardinality;     /* zipf support size                              */
+  double zipf_s;            /* zipf exponent                                  */
+  double run_mean;          /* mean run length (zipf-runs)                     */
+  uint64_t seed;
+} wahgpu_gen_params;
+int wahgpu_generate(const wahgpu_gen_params* p, uint32_t* out, uint64_t n);
+/* Values at positions [start, start + count) of the same column. */
+int wahgpu_generate_range(const wahgpu_gen_params* p, uint32_t* out, uint64_t start,
+                          uint64_t count);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,351 @@
+"""indexing_amd -- MI355X-native WAH bitmap-index construction.
+
+Python face of libwahgpu (include/wahgpu.h), mirroring the stages of the
+reference pipeline (src/cpu.cpp:176-272, src/phases.cpp:720-1088):
+
+    sort_rids_by_value -> produce_chunk_id_literals -> merge_lit_by_val_chids
+    -> produce_fills -> fuse_fill_literals -> compute_column_length
+
+Stage functions take and return torch CUDA tensors (int32 storage holding
+uint32 bit patterns) and run on torch's current stream.  ``Context.build``
+runs the fused build on a host array or a device tensor.  Every call goes to
+the HIP kernels; without the built library the import of a GPU entry point
+raises (there is no CPU fallback).
+"""
+import ctypes
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import _native as N
+from ._native import WahError, MAX_VALUES
+
+__all__ = [
+    "Context", "Index", "WahError", "MAX_VALUES", "Merger", "read_values_text",
+    "read_values_binary", "decode_bitmap", "generate", "device_count",
+]
+
+
+def _torch():
+    import torch
+    return torch
+
+
+def _ptr(t):
+    return ctypes.c_void_p(t.data_ptr()) if t is not None and t.numel() else ctypes.c_void_p(0)
+
+
+def _as_u32(a):
+    return np.ascontiguousarray(a, dtype=np.uint32)
+
+
+@dataclass
+class Index:
+    """A built index: words[offsets[i] : offsets[i] + lengths[i]] encode keys[i]."""
+    keys: np.ndarray
+    offsets: np.ndarray
+    lengths: np.ndarray
+    words: np.ndarray
+    k: int = 0
+    length: int = 0
+    keycnt: int = 0
+    bound: int = 0
+    passes: int = 0
+    times: dict = field(default_factory=dict)
+
+    def key_words(self, i):
+        o = int(self.offsets[i])
+        return self.words[o:o + int(self.lengths[i])]
+
+
+def device_count():
+    c = ctypes.c_int(0)
+    N.lib().wahgpu_device_count(ctypes.byref(c))
+    return c.value
+
+
+class Context:
+    """One HIP stream plus a pooled device arena (wahgpu_create)."""
+
+    def __init__(self, device=0, arena_bytes=0, use_torch_stream=True):
+        self._lib = N.lib()
+        h = ctypes.c_void_p()
+        rc = self._lib.wahgpu_create(ctypes.byref(h), int(device), int(arena_bytes))
+        if rc != N.OK:
+            raise WahError(rc, f"wahgpu_create(device={device}) failed")
+        self._h = h
+        self.device = device
+        self.use_torch_stream = use_torch_stream
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self._lib.wahgpu_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    # -- plumbing ----------------------------------------------------------
+    def _check(self, rc):
+        if rc != N.OK:
+            msg = self._lib.wahgpu_last_error(self._h)
+            raise WahError(rc, msg.decode() if msg else "")
+
+    def _bind_stream(self):
+        """Order library work on torch's current stream."""
+        if self.use_torch_stream:
+            torch = _torch()
+            s = torch.cuda.current_stream(self.device).cuda_stream
+            self._check(self._lib.wahgpu_set_stream(self._h, ctypes.c_void_p(s)))
+
+    @property
+    def stream(self):
+        return self._lib.wahgpu_stream(self._h)
+
+    def synchronize(self):
+        self._check(self._lib.wahgpu_synchronize(self._h))
+
+    def timing_begin(self, max_calls):
+        """Record per-kernel HIP events for the next build_device calls."""
+        self._check(self._lib.wahgpu_timing_begin(self._h, int(max_calls)))
+
+    def timing_end(self):
+        """-> (per-group summed ms, calls); groups: histogram, pass0..pass3,
+        tail, lengths."""
+        ms = (ctypes.c_double * N.TIMING_GROUPS)()
+        calls = ctypes.c_uint32(0)
+        self._check(self._lib.wahgpu_timing_end(self._h, ms, ctypes.byref(calls)))
+        return list(ms), calls.value
+
+    def arena_bytes(self):
+        return int(self._lib.wahgpu_arena_bytes(self._h))
+
+    def _empty(self, n):
+        torch = _torch()
+        return torch.empty(max(int(n), 0), dtype=torch.int32, device=f"cuda:{self.device}")
+
+    # -- whole build ---------------------------------------------------------
+    def build(self, values, rid_base=0, staged=False, timed=False):
+        """Build the index of one column given as a host array (uint32)."""
+        v = _as_u32(values)
+        n = v.size
+        cap = max(n, 1)
+        keys = np.empty(cap, np.uint32)
+        offs = np.empty(cap, np.uint32)
+        lens = np.empty(cap, np.uint32)
+        words = np.empty(2 * cap, np.uint32)
+        c = N.Counts()
+        t = N.PhaseTimes()
+        vp = v.ctypes.data_as(ctypes.c_void_p)
+        args = (keys.ctypes.data_as(ctypes.c_void_p), offs.ctypes.data_as(ctypes.c_void_p),
+                lens.ctypes.data_as(ctypes.c_void_p), cap,
+                words.ctypes.data_as(ctypes.c_void_p), 2 * cap, ctypes.byref(c),
+                ctypes.byref(t) if timed else None)
+        if staged:
+            if rid_base:
+                raise ValueError("the staged build indexes rows from 0")
+            self._check(self._lib.wahgpu_build_staged(self._h, vp, n, *args))
+        else:
+            self._check(self._lib.wahgpu_build(self._h, vp, n, int(rid_base), *args))
+        K, L = int(c.keycnt), int(c.length)
+        times = {name: getattr(t, name) for name, _ in N.PhaseTimes._fields_} if timed else {}
+        return Index(keys[:K].copy(), offs[:K].copy(), lens[:K].copy(), words[:L].copy(),
+                     int(c.k), L, K, int(c.bound), int(c.passes), times)
+
+    def build_device(self, d_values, rid_base=0, out=None):
+        """Fused build on a device tensor; returns (keys, offsets, lengths, words,
+        counts) as device tensors sized to the worst case (n, n, n, 2n) plus a
+        device Counts buffer.  Asynchronous on torch's current stream."""
+        torch = _torch()
+        n = int(d_values.numel())
+        if out is None:
+            cap = max(n, 1)
+            out = (self._empty(cap), self._empty(cap), self._empty(cap), self._empty(2 * cap),
+                   torch.zeros(ctypes.sizeof(N.Counts) // 4, dtype=torch.int32,
+                               device=d_values.device))
+        keys, offs, lens, words, counts = out
+        self._bind_stream()
+        self._check(self._lib.wahgpu_build_device(
+            self._h, _ptr(d_values), n, int(rid_base), _ptr(keys), _ptr(offs), _ptr(lens),
+            _ptr(words), _ptr(counts)))
+        return out
+
+    @staticmethod
+    def read_counts(counts_tensor):
+        raw = counts_tensor.cpu().numpy().view(np.uint8).tobytes()
+        return N.Counts.from_buffer_copy(raw[:ctypes.sizeof(N.Counts)])
+
+    # -- stages (device tensors) -------------------------------------------
+    def sort_rids_by_value(self, values, rid_base=0):
+        n = values.numel()
+        keys, rids = self._empty(n), self._empty(n)
+        self._bind_stream()
+        self._check(self._lib.wahgpu_sort_rids_by_value(self._h, _ptr(values), n,
+                                                        int(rid_base), _ptr(keys), _ptr(rids)))
+        return keys, rids
+
+    def produce_chunk_id_literals(self, rids):
+        n = rids.numel()
+        chids, lits = self._empty(n), self._empty(n)
+        self._bind_stream()
+        self._check(self._lib.wahgpu_produce_chunk_id_literals(self._h, _ptr(rids), n,
+                                                               _ptr(chids), _ptr(lits)))
+        return chids, lits
+
+    def merge_lit_by_val_chids(self, values, chids, lits):
+        n = values.numel()
+        ov, oc, ol = self._empty(n), self._empty(n), self._empty(n)
+        k = ctypes.c_uint64(0)
+        self._bind_stream()
+        self._check(self._lib.wahgpu_merge_lit_by_val_chids(
+            self._h, _ptr(values), _ptr(chids), _ptr(lits), n, _ptr(ov), _ptr(oc), _ptr(ol),
+            ctypes.byref(k)))
+        k = k.value
+        return ov[:k], oc[:k], ol[:k]
+
+    def produce_fills(self, values, chids):
+        k = values.numel()
+        fills = self._empty(k)
+        self._bind_stream()
+        self._check(self._lib.wahgpu_produce_fills(self._h, _ptr(values), _ptr(chids), k,
+                                                   _ptr(fills)))
+        return fills
+
+    def fuse_fill_literals(self, fills, lits):
+        k = fills.numel()
+        index = self._empty(2 * k)
+        L = ctypes.c_uint64(0)
+        self._bind_stream()
+        self._check(self._lib.wahgpu_fuse_fill_literals(self._h, _ptr(fills), _ptr(lits), k,
+                                                        _ptr(index), ctypes.byref(L)))
+        return index[:L.value]
+
+    def compute_column_length(self, values, fills):
+        k = values.numel()
+        keys, offs, lens = self._empty(k), self._empty(k), self._empty(k)
+        K = ctypes.c_uint64(0)
+        self._bind_stream()
+        self._check(self._lib.wahgpu_compute_column_length(
+            self._h, _ptr(values), _ptr(fills), k, _ptr(keys), _ptr(offs), _ptr(lens),
+            ctypes.byref(K)))
+        K = K.value
+        return keys[:K], offs[:K], lens[:K]
+
+    # -- primitives ----------------------------------------------------------
+    def stream_compaction(self, data, valid):
+        n = data.numel()
+        out = self._empty(n)
+        cnt = ctypes.c_uint64(0)
+        self._bind_stream()
+        self._check(self._lib.wahgpu_stream_compaction(self._h, _ptr(data), _ptr(valid), n,
+                                                       _ptr(out), ctypes.byref(cnt)))
+        return out[:cnt.value]
+
+    def exclusive_scan(self, x):
+        n = x.numel()
+        out = self._empty(n)
+        self._bind_stream()
+        self._check(self._lib.wahgpu_exclusive_scan(self._h, _ptr(x), n, _ptr(out)))
+        return out
+
+    def segmented_inclusive_scan(self, x, heads):
+        n = x.numel()
+        out = self._empty(n)
+        self._bind_stream()
+        self._check(self._lib.wahgpu_segmented_inclusive_scan(self._h, _ptr(x), _ptr(heads), n,
+                                                              _ptr(out)))
+        return out
+
+
+# -- host-side formats --------------------------------------------------------
+def _read(fn, path):
+    L = N.lib()
+    p = N.c_u32p()
+    n = ctypes.c_uint64(0)
+    rc = fn(str(path).encode(), ctypes.byref(p), ctypes.byref(n))
+    if rc != N.OK:
+        raise WahError(rc, f"cannot read {path}")
+    try:
+        return np.ctypeslib.as_array(p, shape=(n.value,)).copy() if n.value else \
+            np.empty(0, np.uint32)
+    finally:
+        L.wahgpu_free(p)
+
+
+def read_values_text(path):
+    """Whitespace-separated decimal uint32s, read like `ifstream >> uint32_t`."""
+    return _read(N.lib().wahgpu_read_values_text, path)
+
+
+def read_values_binary(path):
+    return _read(N.lib().wahgpu_read_values_binary, path)
+
+
+def decode_bitmap(words):
+    """'0'/'1' text of a key's bitmap, trailing zeros trimmed (src/cpu.cpp:47-74)."""
+    w = _as_u32(words)
+    L = N.lib()
+    need = L.wahgpu_decode_bitmap(w.ctypes.data_as(ctypes.c_void_p), w.size, None, 0)
+    buf = ctypes.create_string_buffer(max(int(need), 1))
+    L.wahgpu_decode_bitmap(w.ctypes.data_as(ctypes.c_void_p), w.size, buf, need)
+    return buf.raw[:need].decode()
+
+
+def generate(n, dist="uniform", seed=0, lo=0, hi=65535, cardinality=1 << 20, zipf_s=1.2,
+             run_mean=4096.0, start=0):
+    """Seeded synthetic column (the `generate` program's distributions)."""
+    p = N.GenParams()
+    p.dist = {"uniform": N.DIST_UNIFORM, "zipf": N.DIST_ZIPF,
+              "zipf-runs": N.DIST_ZIPF_RUNS}[dist]
+    p.from_, p.to, p.cardinality = lo, hi, cardinality
+    p.zipf_s, p.run_mean, p.seed = zipf_s, run_mean, seed
+    out = np.empty(max(int(n), 1), np.uint32)
+    rc = N.lib().wahgpu_generate_range(ctypes.byref(p), out.ctypes.data_as(ctypes.c_void_p),
+                                       int(start), int(n))
+    if rc != N.OK:
+        raise WahError(rc, "bad generator parameters")
+    return out[:n]
+
+
+class Merger:
+    """Exact stitching of partial indexes over consecutive row ranges."""
+
+    def __init__(self):
+        self._lib = N.lib()
+        h = ctypes.c_void_p()
+        rc = self._lib.wahgpu_merger_create(ctypes.byref(h))
+        if rc != N.OK:
+            raise WahError(rc, "merger")
+        self._h = h
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            self._lib.wahgpu_merger_destroy(self._h)
+            self._h = None
+
+    def add(self, keys, offsets, words):
+        k, o, w = _as_u32(keys), _as_u32(offsets), _as_u32(words)
+        rc = self._lib.wahgpu_merger_add(self._h, k.size, w.size,
+                                         k.ctypes.data_as(ctypes.c_void_p),
+                                         o.ctypes.data_as(ctypes.c_void_p),
+                                         w.ctypes.data_as(ctypes.c_void_p))
+        if rc != N.OK:
+            raise WahError(rc, "malformed partial index")
+
+    def result(self):
+        K = int(self._lib.wahgpu_merger_keycnt(self._h))
+        L = int(self._lib.wahgpu_merger_length(self._h))
+        keys, offs, lens = (np.empty(max(K, 1), np.uint32) for _ in range(3))
+        words = np.empty(max(L, 1), np.uint32)
+        self._lib.wahgpu_merger_export(self._h, *(a.ctypes.data_as(ctypes.c_void_p)
+                                                  for a in (keys, offs, lens, words)))
+        return Index(keys[:K], offs[:K], lens[:K], words[:L], 0, L, K)

@@ -1,0 +1,176 @@
+// wah_device.h -- device-side building blocks for the gfx950 WAH kernels:
+// wave64 intrinsics, block scans over 256 threads, and the decoupled
+// look-back that chains tiles of a single-pass scan across the whole grid.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace wah {
+
+constexpr int kWave = 64;
+constexpr int kBlock = 256;               // every kernel here runs 4 waves
+constexpr int kWaves = kBlock / kWave;
+
+// Look-back status flags (top two bits of a status word).
+constexpr uint32_t kFlagAgg = 1u;         // tile aggregate published
+constexpr uint32_t kFlagPrefix = 2u;      // inclusive prefix published
+
+__device__ __forceinline__ uint32_t lane_id() {
+  return __builtin_amdgcn_mbcnt_hi(~0u, __builtin_amdgcn_mbcnt_lo(~0u, 0u));
+}
+__device__ __forceinline__ uint64_t ballot(bool p) {
+  return __builtin_amdgcn_ballot_w64(p);
+}
+// Number of set bits of m below this lane.
+__device__ __forceinline__ uint32_t mbcnt(uint64_t m) {
+  return __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32),
+                                   __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+}
+
+// Agent-scope relaxed atomics: status words are read by blocks on other XCDs,
+// whose L2 is not this block's L2.  The word itself carries flag and value,
+// so no ordering with other memory is needed.
+__device__ __forceinline__ uint32_t ld_agent(uint32_t* p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void st_agent(uint32_t* p, uint32_t v) {
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ uint64_t ld_agent(uint64_t* p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void st_agent(uint64_t* p, uint64_t v) {
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+__device__ __forceinline__ uint64_t shfl_up64(uint64_t x, int d) {
+  uint32_t lo = __shfl_up((uint32_t)x, d, kWave);
+  uint32_t hi = __shfl_up((uint32_t)(x >> 32), d, kWave);
+  return ((uint64_t)hi << 32) | lo;
+}
+__device__ __forceinline__ uint64_t shfl64(uint64_t x, int src) {
+  uint32_t lo = __shfl((uint32_t)x, src, kWave);
+  uint32_t hi = __shfl((uint32_t)(x >> 32), src, kWave);
+  return ((uint64_t)hi << 32) | lo;
+}
+
+// ---- associative operators on 62-bit look-back payloads -------------------
+// combine(a, b): a covers the earlier elements, b the later ones.
+
+// Two independent 31-bit counters packed as (hi << 31 | lo); per-operand
+// values stay below 2^31 because a column holds fewer than 2^30 values.
+struct OpAdd {
+  static __device__ __forceinline__ uint64_t identity() { return 0; }
+  static __device__ __forceinline__ uint64_t combine(uint64_t a, uint64_t b) { return a + b; }
+};
+// uint32 sum that wraps mod 2^32 (the reference's uint32 scans).
+struct OpAdd32 {
+  static __device__ __forceinline__ uint64_t identity() { return 0; }
+  static __device__ __forceinline__ uint64_t combine(uint64_t a, uint64_t b) {
+    return (uint32_t)(a + b);
+  }
+};
+// Segmented OR: payload = heads (30 bits) << 32 | or-since-last-head (32).
+struct OpSegOr {
+  static __device__ __forceinline__ uint64_t identity() { return 0; }
+  static __device__ __forceinline__ uint64_t combine(uint64_t a, uint64_t b) {
+    uint64_t h = (a >> 32) + (b >> 32);
+    uint32_t x = (b >> 32) ? (uint32_t)b : ((uint32_t)a | (uint32_t)b);
+    return (h << 32) | x;
+  }
+};
+// Segmented sum: payload = has-head (bit 32) | sum-since-last-head (32).
+struct OpSegAdd {
+  static __device__ __forceinline__ uint64_t identity() { return 0; }
+  static __device__ __forceinline__ uint64_t combine(uint64_t a, uint64_t b) {
+    bool hb = (b >> 32) & 1u;
+    uint32_t x = hb ? (uint32_t)b : (uint32_t)a + (uint32_t)b;
+    return ((uint64_t)(((a >> 32) | (b >> 32)) & 1u) << 32) | x;
+  }
+};
+
+// Inclusive scan within a wave.
+template <class Op>
+__device__ __forceinline__ uint64_t wave_inclusive(uint64_t x) {
+  const uint32_t l = lane_id();
+#pragma unroll
+  for (int d = 1; d < kWave; d <<= 1) {
+    uint64_t y = shfl_up64(x, d);
+    if (l >= (uint32_t)d) x = Op::combine(y, x);
+  }
+  return x;
+}
+
+// Exclusive scan over the 256 threads of a block; *total receives the block
+// aggregate.  smem holds kWaves entries.  Contains __syncthreads().
+template <class Op>
+__device__ __forceinline__ uint64_t block_exclusive(uint64_t x, uint64_t* smem,
+                                                    uint64_t* total) {
+  const int t = threadIdx.x, w = t / kWave, l = t % kWave;
+  uint64_t inc = wave_inclusive<Op>(x);
+  uint64_t exc = shfl_up64(inc, 1);
+  if (l == 0) exc = Op::identity();
+  if (l == kWave - 1) smem[w] = inc;
+  __syncthreads();
+  uint64_t before = Op::identity();
+  uint64_t all = Op::identity();
+#pragma unroll
+  for (int i = 0; i < kWaves; ++i) {
+    if (i < w) before = Op::combine(before, smem[i]);
+    all = Op::combine(all, smem[i]);
+  }
+  __syncthreads();
+  *total = all;
+  return Op::combine(before, exc);
+}
+
+// Bounded wait on a look-back status word: returns the word once its flag is
+// set, or 0 after kSpinLimit polls (then *fault is raised and the caller
+// proceeds with a zero contribution, so every wave still finishes).
+// Once any wait of the launch has given up, the others stop waiting too.
+constexpr uint32_t kSpinLimit = 1u << 18;
+
+template <class T, int SHIFT>
+__device__ __forceinline__ T wait_status_t(T* p, uint32_t* fault) {
+  T s;
+  for (uint32_t i = 0; i < kSpinLimit; ++i) {
+    if ((s = ld_agent(p)) >> SHIFT) return s;
+    if ((i & 1023u) == 1023u && ld_agent(fault)) return 0;
+    __builtin_amdgcn_s_sleep(1);
+  }
+  atomicOr(fault, 1u);
+  return 0;
+}
+__device__ __forceinline__ uint32_t wait_status(uint32_t* p, uint32_t* fault) {
+  return wait_status_t<uint32_t, 30>(p, fault);
+}
+__device__ __forceinline__ uint64_t wait_status(uint64_t* p, uint32_t* fault) {
+  return wait_status_t<uint64_t, 62>(p, fault);
+}
+
+// Decoupled look-back over 64-bit status words (flag << 62 | payload).
+// Called by one thread of the tile; publishes the tile's aggregate first so
+// later tiles can proceed, then walks back to the nearest inclusive prefix,
+// publishes its own prefix and returns the exclusive prefix of the tile.
+template <class Op>
+__device__ uint64_t lookback64(uint64_t* status, uint32_t tile, uint64_t agg, uint32_t* fault) {
+  constexpr uint64_t kMask = (1ull << 62) - 1;
+  if (tile == 0) {
+    st_agent(&status[0], ((uint64_t)kFlagPrefix << 62) | (agg & kMask));
+    return Op::identity();
+  }
+  st_agent(&status[tile], ((uint64_t)kFlagAgg << 62) | (agg & kMask));
+  uint64_t acc = Op::identity();
+  int64_t t = (int64_t)tile - 1;
+  while (t >= 0) {
+    uint64_t s = wait_status(&status[t], fault);
+    acc = Op::combine(s & kMask, acc);
+    if ((s >> 62) != kFlagAgg) break;  // prefix found (or gave up)
+    --t;
+  }
+  st_agent(&status[tile], ((uint64_t)kFlagPrefix << 62) | (Op::combine(acc, agg) & kMask));
+  return acc;
+}
+
+}  // namespace wah

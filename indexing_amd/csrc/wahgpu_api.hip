@@ -1,0 +1,722 @@
+// wahgpu_api.hip -- the C ABI of libwahgpu (include/wahgpu.h): contexts with a
+// stream and a pooled device arena, the fused and staged builds, the stage
+// entry points and the scan / compaction primitives.
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <stdio.h>
+#include <string.h>
+
+#include <chrono>
+#include <string>
+#include <vector>
+
+#include "wah_kernels.h"
+#include "wahgpu.h"
+
+using wah::BuildArgs;
+using wah::ScanArgs;
+
+struct wahgpu_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;      // stream all work is ordered on
+  hipStream_t own_stream = nullptr;  // created with the context
+  char* arena = nullptr;
+  uint64_t arena_cap = 0;
+  bool fixed = false;
+  uint64_t bump = 0;
+  uint32_t max_resident = 1024;
+  std::string err;
+  wahgpu_counts* pinned_counts = nullptr;  // host-pinned landing slot
+  // per-kernel timing (wahgpu_timing_begin/end)
+  std::vector<hipEvent_t> tev;  // max_calls x (WAHGPU_TIMING_GROUPS + 1)
+  uint32_t tmax = 0, tcalls = 0;
+  bool timing = false;
+};
+
+namespace {
+
+int fail(wahgpu_ctx* c, int code, const std::string& msg) {
+  if (c) c->err = msg;
+  return code;
+}
+
+#define HIPCHK(ctx, call)                                                             \
+  do {                                                                                \
+    hipError_t e_ = (call);                                                           \
+    if (e_ != hipSuccess)                                                             \
+      return fail(ctx, WAHGPU_EHIP, std::string(#call) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+#define LAUNCHCHK(ctx)                                                                 \
+  do {                                                                                 \
+    hipError_t e_ = hipGetLastError();                                                 \
+    if (e_ != hipSuccess)                                                              \
+      return fail(ctx, WAHGPU_EHIP, std::string("kernel launch: ") + hipGetErrorString(e_)); \
+  } while (0)
+
+constexpr uint64_t kAlign = 256;
+uint64_t align_up(uint64_t x) { return (x + kAlign - 1) / kAlign * kAlign; }
+
+// Reserve `bytes` of arena for the current call (grows unless fixed).
+int arena_reserve(wahgpu_ctx* c, uint64_t bytes) {
+  c->bump = 0;
+  if (bytes <= c->arena_cap) return WAHGPU_OK;
+  if (c->fixed)
+    return fail(c, WAHGPU_ENOMEM, "arena of " + std::to_string(c->arena_cap) +
+                                      " bytes is too small for " + std::to_string(bytes));
+  HIPCHK(c, hipSetDevice(c->device));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  if (c->arena) {
+    (void)hipFree(c->arena);
+    c->arena = nullptr;
+    c->arena_cap = 0;
+  }
+  uint64_t cap = (bytes + (2u << 20) - 1) / (2u << 20) * (2u << 20);
+  if (hipMalloc(&c->arena, cap) != hipSuccess) {
+    (void)hipGetLastError();
+    c->arena = nullptr;
+    return fail(c, WAHGPU_ENOMEM, "hipMalloc of " + std::to_string(cap) + " bytes failed");
+  }
+  c->arena_cap = cap;
+  return WAHGPU_OK;
+}
+
+template <class T>
+T* take(wahgpu_ctx* c, uint64_t count) {
+  T* p = reinterpret_cast<T*>(c->arena + c->bump);
+  c->bump += align_up(count * sizeof(T));
+  return p;
+}
+
+// Bytes of scratch a fused build of n values needs (besides outputs).
+uint64_t build_scratch_bytes(uint64_t n) {
+  uint64_t b = 0;
+  b += 4 * align_up(n * 4);                                  // keysA ridsA keysB ridsB
+  b += align_up(sizeof(wah::SortState));
+  b += align_up(wah::kHistMaxBlocks * 1024 * 4);
+  b += align_up(wah::kHistMaxBlocks * 4);
+  b += align_up(wah::sort_status_words(n) * 4);
+  b += align_up(wah::tail_status_words(n) * 8);
+  return b;
+}
+
+void carve_build(wahgpu_ctx* c, BuildArgs* a, uint64_t n) {
+  a->keysA = take<uint32_t>(c, n);
+  a->ridsA = take<uint32_t>(c, n);
+  a->keysB = take<uint32_t>(c, n);
+  a->ridsB = take<uint32_t>(c, n);
+  a->st = take<wah::SortState>(c, 1);
+  a->part_hist = take<uint32_t>(c, wah::kHistMaxBlocks * 1024);
+  a->part_max = take<uint32_t>(c, wah::kHistMaxBlocks);
+  a->sort_status = take<uint32_t>(c, wah::sort_status_words(n));
+  a->tail_status = take<uint64_t>(c, wah::tail_status_words(n));
+  a->max_resident = c->max_resident;
+}
+
+// Scan scratch: [counter u32, pad, result u64, status u64 x tiles] zeroed.
+struct ScanScratch {
+  uint32_t* base;
+  uint64_t words;
+  ScanArgs args;
+  uint64_t* result;
+};
+uint64_t scan_scratch_bytes(uint64_t n) { return align_up(16 + wah::scan_status_words(n) * 8); }
+ScanScratch carve_scan(wahgpu_ctx* c, uint64_t n) {
+  ScanScratch s;
+  uint64_t bytes = 16 + wah::scan_status_words(n) * 8;
+  s.base = take<uint32_t>(c, bytes / 4);
+  s.words = bytes / 4;
+  s.args.tile_counter = s.base;
+  s.args.fault = s.base + 1;
+  s.result = reinterpret_cast<uint64_t*>(s.base + 2);
+  s.args.status = reinterpret_cast<uint64_t*>(s.base + 4);
+  s.args.max_resident = c->max_resident;
+  return s;
+}
+
+int check_n(wahgpu_ctx* c, uint64_t n, uint32_t rid_base) {
+  if (n > WAHGPU_MAX_VALUES)
+    return fail(c, WAHGPU_ERANGE, "column of " + std::to_string(n) + " values exceeds " +
+                                      std::to_string(WAHGPU_MAX_VALUES));
+  if ((uint64_t)rid_base + n > ((uint64_t)1 << 32))
+    return fail(c, WAHGPU_ERANGE, "rid_base + n exceeds 2^32 rows");
+  return WAHGPU_OK;
+}
+
+int read_counts(wahgpu_ctx* c, const wahgpu_counts* d_counts, wahgpu_counts* out) {
+  HIPCHK(c, hipMemcpyAsync(c->pinned_counts, d_counts, sizeof(wahgpu_counts),
+                           hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  *out = *c->pinned_counts;
+  if (out->fault)
+    return fail(c, WAHGPU_EHIP, "kernel reported fault " + std::to_string(out->fault) +
+                                    " (look-back wait gave up or index out of range)");
+  return WAHGPU_OK;
+}
+
+// Reads a scan scratch header [counter, fault, result] and its result.
+int read_scan_result(wahgpu_ctx* c, const uint32_t* base, uint64_t* out) {
+  HIPCHK(c, hipMemcpyAsync(c->pinned_counts, base, 16, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  uint32_t hdr[4];
+  memcpy(hdr, c->pinned_counts, 16);
+  if (hdr[1])
+    return fail(c, WAHGPU_EHIP, "kernel reported fault " + std::to_string(hdr[1]) +
+                                    " (look-back wait gave up or index out of range)");
+  memcpy(out, &hdr[2], 8);
+  return WAHGPU_OK;
+}
+
+// Events bracketing phases; elapsed microseconds between consecutive marks.
+struct Marks {
+  hipEvent_t ev[12];
+  int n = 0;
+  bool on = false;
+  void init(bool enable) {
+    on = enable;
+    if (on)
+      for (auto& e : ev) (void)hipEventCreate(&e);
+  }
+  void mark(hipStream_t s) {
+    if (on && n < 12) (void)hipEventRecord(ev[n++], s);
+  }
+  double us(int a, int b) {
+    if (!on || b >= n) return 0.0;
+    float ms = 0.f;
+    (void)hipEventElapsedTime(&ms, ev[a], ev[b]);
+    return ms * 1000.0;
+  }
+  ~Marks() {
+    if (on)
+      for (auto& e : ev) (void)hipEventDestroy(e);
+  }
+};
+
+}  // namespace
+
+extern "C" {
+
+int wahgpu_device_count(int* count) {
+  int c = 0;
+  if (hipGetDeviceCount(&c) != hipSuccess) {
+    (void)hipGetLastError();
+    c = 0;
+  }
+  if (count) *count = c;
+  return c > 0 ? WAHGPU_OK : WAHGPU_ENODEV;
+}
+
+int wahgpu_device_name(int ordinal, char* buf, size_t cap) {
+  hipDeviceProp_t prop;
+  if (hipGetDeviceProperties(&prop, ordinal) != hipSuccess) {
+    (void)hipGetLastError();
+    return WAHGPU_ENODEV;
+  }
+  if (buf && cap) {
+    snprintf(buf, cap, "%s", prop.name);
+  }
+  return WAHGPU_OK;
+}
+
+int wahgpu_device_compute_units(int ordinal) {
+  hipDeviceProp_t prop;
+  if (hipGetDeviceProperties(&prop, ordinal) != hipSuccess) {
+    (void)hipGetLastError();
+    return WAHGPU_ENODEV;
+  }
+  return prop.multiProcessorCount;
+}
+
+int wahgpu_find_device(const char* name) {
+  int count = 0;
+  wahgpu_device_count(&count);
+  for (int i = 0; i < count; ++i) {
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, i) != hipSuccess) continue;
+    if (!name || !*name) return i;
+    std::string arch(prop.gcnArchName);
+    arch = arch.substr(0, arch.find(':'));
+    if (std::string(prop.name) == name || arch == name) return i;
+  }
+  return -1;
+}
+
+int wahgpu_create(wahgpu_ctx** out, int device_ordinal, uint64_t arena_bytes) {
+  if (!out) return WAHGPU_EINVAL;
+  *out = nullptr;
+  int count = 0;
+  wahgpu_device_count(&count);
+  if (device_ordinal < 0 || device_ordinal >= count) return WAHGPU_ENODEV;
+  wahgpu_ctx* c = new wahgpu_ctx();
+  c->device = device_ordinal;
+  if (hipSetDevice(device_ordinal) != hipSuccess ||
+      hipStreamCreateWithFlags(&c->own_stream, hipStreamNonBlocking) != hipSuccess ||
+      hipHostMalloc(reinterpret_cast<void**>(&c->pinned_counts), 64, hipHostMallocDefault) !=
+          hipSuccess) {
+    (void)hipGetLastError();
+    delete c;
+    return WAHGPU_EHIP;
+  }
+  c->stream = c->own_stream;
+  hipDeviceProp_t prop;
+  if (hipGetDeviceProperties(&prop, device_ordinal) == hipSuccess)
+    c->max_resident = (uint32_t)prop.multiProcessorCount * 4u;
+  if (arena_bytes) {
+    c->fixed = false;
+    int rc = arena_reserve(c, arena_bytes);
+    if (rc != WAHGPU_OK) {
+      wahgpu_destroy(c);
+      return rc;
+    }
+    c->fixed = true;
+  }
+  *out = c;
+  return WAHGPU_OK;
+}
+
+void wahgpu_destroy(wahgpu_ctx* c) {
+  if (!c) return;
+  (void)hipSetDevice(c->device);
+  for (auto e : c->tev) (void)hipEventDestroy(e);
+  if (c->stream) (void)hipStreamSynchronize(c->stream);
+  if (c->arena) (void)hipFree(c->arena);
+  if (c->pinned_counts) (void)hipHostFree(c->pinned_counts);
+  if (c->own_stream) (void)hipStreamDestroy(c->own_stream);
+  delete c;
+}
+
+const char* wahgpu_last_error(const wahgpu_ctx* c) { return c ? c->err.c_str() : ""; }
+void* wahgpu_stream(wahgpu_ctx* c) { return c ? (void*)c->stream : nullptr; }
+
+int wahgpu_set_stream(wahgpu_ctx* c, void* stream) {
+  if (!c) return WAHGPU_EINVAL;
+  c->stream = stream ? (hipStream_t)stream : c->own_stream;
+  return WAHGPU_OK;
+}
+uint64_t wahgpu_arena_bytes(const wahgpu_ctx* c) { return c ? c->arena_cap : 0; }
+
+int wahgpu_synchronize(wahgpu_ctx* c) {
+  if (!c) return WAHGPU_EINVAL;
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return WAHGPU_OK;
+}
+
+// ---- fused build ---------------------------------------------------------
+
+namespace {
+struct EventHook : wah::LaunchHook {
+  hipEvent_t* ev;
+  int n = 0;
+  void mark(hipStream_t s) override { (void)hipEventRecord(ev[n++], s); }
+};
+}  // namespace
+
+static int build_device_impl(wahgpu_ctx* c, const uint32_t* d_values, uint64_t n,
+                             uint32_t rid_base, uint32_t* d_keys, uint32_t* d_offsets,
+                             uint32_t* d_lengths, uint32_t* d_words, wahgpu_counts* d_counts,
+                             Marks* marks) {
+  BuildArgs a{};
+  carve_build(c, &a, n);
+  a.values = d_values;
+  a.n = n;
+  a.rid_base = rid_base;
+  a.out_keys = d_keys;
+  a.out_offsets = d_offsets;
+  a.out_lengths = d_lengths;
+  a.out_words = d_words;
+  a.counts = d_counts;
+  EventHook hook;
+  EventHook* hp = nullptr;
+  if (c->timing && c->tcalls < c->tmax) {
+    hook.ev = &c->tev[(size_t)c->tcalls * (WAHGPU_TIMING_GROUPS + 1)];
+    hp = &hook;
+    ++c->tcalls;
+    hook.mark(c->stream);
+  }
+  wah::launch_sort(a, c->stream, hp);
+  LAUNCHCHK(c);
+  if (marks) marks->mark(c->stream);
+  wah::launch_tail(a, c->stream, hp);
+  LAUNCHCHK(c);
+  if (hp) hook.mark(c->stream);
+  return WAHGPU_OK;
+}
+
+int wahgpu_timing_begin(wahgpu_ctx* c, uint32_t max_calls) {
+  if (!c || !max_calls) return WAHGPU_EINVAL;
+  HIPCHK(c, hipSetDevice(c->device));
+  for (auto e : c->tev) (void)hipEventDestroy(e);
+  c->tev.assign((size_t)max_calls * (WAHGPU_TIMING_GROUPS + 1), nullptr);
+  for (auto& e : c->tev) HIPCHK(c, hipEventCreate(&e));
+  c->tmax = max_calls;
+  c->tcalls = 0;
+  c->timing = true;
+  return WAHGPU_OK;
+}
+
+int wahgpu_timing_end(wahgpu_ctx* c, double* ms, uint32_t* calls) {
+  if (!c || !ms) return WAHGPU_EINVAL;
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  for (int g = 0; g < WAHGPU_TIMING_GROUPS; ++g) ms[g] = 0.0;
+  for (uint32_t i = 0; i < c->tcalls; ++i) {
+    hipEvent_t* ev = &c->tev[(size_t)i * (WAHGPU_TIMING_GROUPS + 1)];
+    for (int g = 0; g < WAHGPU_TIMING_GROUPS; ++g) {
+      float t = 0.f;
+      HIPCHK(c, hipEventElapsedTime(&t, ev[g], ev[g + 1]));
+      ms[g] += t;
+    }
+  }
+  if (calls) *calls = c->tcalls;
+  c->timing = false;
+  for (auto e : c->tev) (void)hipEventDestroy(e);
+  c->tev.clear();
+  c->tcalls = c->tmax = 0;
+  return WAHGPU_OK;
+}
+
+int wahgpu_build_device(wahgpu_ctx* c, const uint32_t* d_values, uint64_t n, uint32_t rid_base,
+                        uint32_t* d_keys, uint32_t* d_offsets, uint32_t* d_lengths,
+                        uint32_t* d_words, wahgpu_counts* d_counts) {
+  if (!c || !d_counts) return fail(c, WAHGPU_EINVAL, "null context or counts");
+  if (n && (!d_values || !d_keys || !d_offsets || !d_lengths || !d_words))
+    return fail(c, WAHGPU_EINVAL, "null buffer");
+  int rc = check_n(c, n, rid_base);
+  if (rc) return rc;
+  HIPCHK(c, hipSetDevice(c->device));
+  rc = arena_reserve(c, build_scratch_bytes(n));
+  if (rc) return rc;
+  return build_device_impl(c, d_values, n, rid_base, d_keys, d_offsets, d_lengths, d_words,
+                           d_counts, nullptr);
+}
+
+int wahgpu_build(wahgpu_ctx* c, const uint32_t* values, uint64_t n, uint32_t rid_base,
+                 uint32_t* keys, uint32_t* offsets, uint32_t* lengths, uint64_t keys_cap,
+                 uint32_t* words, uint64_t words_cap, wahgpu_counts* counts,
+                 wahgpu_phase_times* times) {
+  if (!c || !counts) return fail(c, WAHGPU_EINVAL, "null context or counts");
+  if (n && !values) return fail(c, WAHGPU_EINVAL, "null values");
+  int rc = check_n(c, n, rid_base);
+  if (rc) return rc;
+  HIPCHK(c, hipSetDevice(c->device));
+  uint64_t nn = n ? n : 1;
+  uint64_t bytes = build_scratch_bytes(n) + align_up(nn * 4) * 4 + align_up(2 * nn * 4) +
+                   align_up(sizeof(wahgpu_counts));
+  rc = arena_reserve(c, bytes);
+  if (rc) return rc;
+  auto t0 = std::chrono::steady_clock::now();
+  uint32_t* d_values = take<uint32_t>(c, nn);
+  uint32_t* d_keys = take<uint32_t>(c, nn);
+  uint32_t* d_offsets = take<uint32_t>(c, nn);
+  uint32_t* d_lengths = take<uint32_t>(c, nn);
+  uint32_t* d_words = take<uint32_t>(c, 2 * nn);
+  wahgpu_counts* d_counts = take<wahgpu_counts>(c, 1);
+  Marks m;
+  m.init(times != nullptr);
+  m.mark(c->stream);
+  if (n) HIPCHK(c, hipMemcpyAsync(d_values, values, n * 4, hipMemcpyHostToDevice, c->stream));
+  m.mark(c->stream);
+  rc = build_device_impl(c, d_values, n, rid_base, d_keys, d_offsets, d_lengths, d_words,
+                         d_counts, &m);
+  if (rc) return rc;
+  m.mark(c->stream);
+  rc = read_counts(c, d_counts, counts);
+  if (rc) return rc;
+  if (counts->keycnt > keys_cap || counts->length > words_cap)
+    return fail(c, WAHGPU_EINVAL, "output capacity too small");
+  if (counts->length && !words) return fail(c, WAHGPU_EINVAL, "null words");
+  if (counts->keycnt && (!keys || !offsets || !lengths))
+    return fail(c, WAHGPU_EINVAL, "null keys/offsets/lengths");
+  if (counts->length)
+    HIPCHK(c, hipMemcpyAsync(words, d_words, counts->length * 4, hipMemcpyDeviceToHost,
+                             c->stream));
+  if (counts->keycnt) {
+    HIPCHK(c, hipMemcpyAsync(keys, d_keys, counts->keycnt * 4, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipMemcpyAsync(offsets, d_offsets, counts->keycnt * 4, hipMemcpyDeviceToHost,
+                             c->stream));
+    HIPCHK(c, hipMemcpyAsync(lengths, d_lengths, counts->keycnt * 4, hipMemcpyDeviceToHost,
+                             c->stream));
+  }
+  m.mark(c->stream);
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  if (times) {
+    memset(times, 0, sizeof(*times));
+    times->transfer_to = m.us(0, 1);
+    times->sort = m.us(1, 2);
+    times->column_length = m.us(2, 3);  // fused tail: chunks..column length
+    times->transfer_back = m.us(3, 4);
+    times->total = std::chrono::duration<double, std::micro>(std::chrono::steady_clock::now() - t0)
+                       .count();
+  }
+  return WAHGPU_OK;
+}
+
+// ---- staged build (one kernel group per reference stage) ------------------
+
+int wahgpu_build_staged(wahgpu_ctx* c, const uint32_t* values, uint64_t n, uint32_t* keys,
+                        uint32_t* offsets, uint32_t* lengths, uint64_t keys_cap,
+                        uint32_t* words, uint64_t words_cap, wahgpu_counts* counts,
+                        wahgpu_phase_times* times) {
+  if (!c || !counts) return fail(c, WAHGPU_EINVAL, "null context or counts");
+  if (n && !values) return fail(c, WAHGPU_EINVAL, "null values");
+  int rc = check_n(c, n, 0);
+  if (rc) return rc;
+  HIPCHK(c, hipSetDevice(c->device));
+  uint64_t nn = n ? n : 1;
+  uint64_t bytes = build_scratch_bytes(n) + align_up(nn * 4) * 12 + align_up(2 * nn * 4) +
+                   align_up(sizeof(wahgpu_counts)) + 4 * scan_scratch_bytes(2 * nn);
+  rc = arena_reserve(c, bytes);
+  if (rc) return rc;
+  auto t0 = std::chrono::steady_clock::now();
+  BuildArgs a{};
+  carve_build(c, &a, n);
+  uint32_t* d_values = take<uint32_t>(c, nn);
+  uint32_t* s_keys = take<uint32_t>(c, nn);
+  uint32_t* s_rids = take<uint32_t>(c, nn);
+  uint32_t* chids = take<uint32_t>(c, nn);
+  uint32_t* lits = take<uint32_t>(c, nn);
+  uint32_t* mv = take<uint32_t>(c, nn);
+  uint32_t* mc = take<uint32_t>(c, nn);
+  uint32_t* ml = take<uint32_t>(c, nn);
+  uint32_t* fills = take<uint32_t>(c, nn);
+  uint32_t* o_keys = take<uint32_t>(c, nn);
+  uint32_t* o_offsets = take<uint32_t>(c, nn);
+  uint32_t* o_lengths = take<uint32_t>(c, nn);
+  uint32_t* index = take<uint32_t>(c, 2 * nn);
+  wahgpu_counts* d_counts = take<wahgpu_counts>(c, 1);
+  ScanScratch sm = carve_scan(c, nn), sf = carve_scan(c, 2 * nn), sc = carve_scan(c, nn);
+  Marks m;
+  m.init(times != nullptr);
+  m.mark(c->stream);  // 0
+  if (n) HIPCHK(c, hipMemcpyAsync(d_values, values, n * 4, hipMemcpyHostToDevice, c->stream));
+  m.mark(c->stream);  // 1 transfer to
+  a.values = d_values;
+  a.n = n;
+  a.rid_base = 0;
+  a.counts = d_counts;
+  a.sorted_keys = s_keys;
+  a.sorted_rids = s_rids;
+  // The passes ping-pong between (keysA, ridsA) and (keysB, ridsB); finalize
+  // copies whichever holds the result into (s_keys, s_rids).
+  wah::launch_sort(a, c->stream);
+  wah::launch_sort_finalize(a, c->stream);
+  LAUNCHCHK(c);
+  m.mark(c->stream);  // 2 sort
+  wah::launch_chunks(s_rids, n, chids, lits, c->stream);
+  LAUNCHCHK(c);
+  m.mark(c->stream);  // 3 chunks + literals
+  wah::launch_fill32(sm.base, sm.words, 0, c->stream);
+  wah::launch_merge(sm.args, s_keys, chids, lits, n, mv, mc, ml, sm.result, c->stream);
+  LAUNCHCHK(c);
+  uint64_t k = 0;
+  rc = read_scan_result(c, sm.base, &k);  // k sizes the next launches (src/phases.cpp:890-903)
+  if (rc) return rc;
+  m.mark(c->stream);  // 4 merge
+  wah::launch_fills(mv, mc, k, fills, c->stream);
+  LAUNCHCHK(c);
+  m.mark(c->stream);  // 5 fills
+  wah::launch_fill32(sf.base, sf.words, 0, c->stream);
+  wah::launch_compact(sf.args, fills, ml, k, 1, index, sf.result, c->stream);
+  LAUNCHCHK(c);
+  uint64_t L = 0;
+  rc = read_scan_result(c, sf.base, &L);
+  if (rc) return rc;
+  m.mark(c->stream);  // 6 fuse
+  wah::launch_fill32(sc.base, sc.words, 0, c->stream);
+  wah::launch_colen(sc.args, mv, fills, k, o_keys, o_offsets, o_lengths, d_counts, c->stream);
+  LAUNCHCHK(c);
+  wahgpu_counts cc;
+  uint64_t unused;
+  rc = read_scan_result(c, sc.base, &unused);  // fault word of the column kernel
+  if (rc) return rc;
+  rc = read_counts(c, d_counts, &cc);
+  if (rc) return rc;
+  m.mark(c->stream);  // 7 column length
+  counts->k = k;
+  counts->length = L;
+  counts->keycnt = cc.keycnt;
+  counts->bound = cc.bound;
+  counts->passes = cc.passes;
+  if (counts->keycnt > keys_cap || counts->length > words_cap)
+    return fail(c, WAHGPU_EINVAL, "output capacity too small");
+  if (L) HIPCHK(c, hipMemcpyAsync(words, index, L * 4, hipMemcpyDeviceToHost, c->stream));
+  if (cc.keycnt) {
+    HIPCHK(c, hipMemcpyAsync(keys, o_keys, cc.keycnt * 4, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipMemcpyAsync(offsets, o_offsets, cc.keycnt * 4, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipMemcpyAsync(lengths, o_lengths, cc.keycnt * 4, hipMemcpyDeviceToHost, c->stream));
+  }
+  m.mark(c->stream);  // 8 transfer back
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  if (times) {
+    times->transfer_to = m.us(0, 1);
+    times->sort = m.us(1, 2);
+    times->chunks = m.us(2, 3);
+    times->merge = m.us(3, 4);
+    times->fills = m.us(4, 5);
+    times->fuse = m.us(5, 6);
+    times->column_length = m.us(6, 7);
+    times->transfer_back = m.us(7, 8);
+    times->total = std::chrono::duration<double, std::micro>(std::chrono::steady_clock::now() - t0)
+                       .count();
+  }
+  return WAHGPU_OK;
+}
+
+// ---- stages ----------------------------------------------------------------
+
+int wahgpu_sort_rids_by_value(wahgpu_ctx* c, const uint32_t* d_values, uint64_t n,
+                              uint32_t rid_base, uint32_t* d_keys_out, uint32_t* d_rids_out) {
+  if (!c) return WAHGPU_EINVAL;
+  if (n && (!d_values || !d_keys_out || !d_rids_out))
+    return fail(c, WAHGPU_EINVAL, "null buffer");
+  int rc = check_n(c, n, rid_base);
+  if (rc) return rc;
+  HIPCHK(c, hipSetDevice(c->device));
+  rc = arena_reserve(c, build_scratch_bytes(n) + align_up(sizeof(wahgpu_counts)));
+  if (rc) return rc;
+  BuildArgs a{};
+  carve_build(c, &a, n);
+  a.values = d_values;
+  a.n = n;
+  a.rid_base = rid_base;
+  a.counts = take<wahgpu_counts>(c, 1);
+  a.sorted_keys = d_keys_out;
+  a.sorted_rids = d_rids_out;
+  wah::launch_sort(a, c->stream);
+  wah::launch_sort_finalize(a, c->stream);
+  LAUNCHCHK(c);
+  return WAHGPU_OK;
+}
+
+int wahgpu_produce_chunk_id_literals(wahgpu_ctx* c, const uint32_t* d_rids, uint64_t n,
+                                     uint32_t* d_chids, uint32_t* d_lits) {
+  if (!c) return WAHGPU_EINVAL;
+  if (n && (!d_rids || !d_chids || !d_lits)) return fail(c, WAHGPU_EINVAL, "null buffer");
+  HIPCHK(c, hipSetDevice(c->device));
+  wah::launch_chunks(d_rids, n, d_chids, d_lits, c->stream);
+  LAUNCHCHK(c);
+  return WAHGPU_OK;
+}
+
+int wahgpu_merge_lit_by_val_chids(wahgpu_ctx* c, const uint32_t* d_values,
+                                  const uint32_t* d_chids, const uint32_t* d_lits, uint64_t n,
+                                  uint32_t* d_values_out, uint32_t* d_chids_out,
+                                  uint32_t* d_lits_out, uint64_t* k) {
+  if (!c || !k) return WAHGPU_EINVAL;
+  if (n && (!d_values || !d_chids || !d_lits || !d_values_out || !d_chids_out || !d_lits_out))
+    return fail(c, WAHGPU_EINVAL, "null buffer");
+  int rc = check_n(c, n, 0);
+  if (rc) return rc;
+  HIPCHK(c, hipSetDevice(c->device));
+  rc = arena_reserve(c, scan_scratch_bytes(n));
+  if (rc) return rc;
+  ScanScratch s = carve_scan(c, n);
+  wah::launch_fill32(s.base, s.words, 0, c->stream);
+  wah::launch_merge(s.args, d_values, d_chids, d_lits, n, d_values_out, d_chids_out, d_lits_out,
+                    s.result, c->stream);
+  LAUNCHCHK(c);
+  return read_scan_result(c, s.base, k);
+}
+
+int wahgpu_produce_fills(wahgpu_ctx* c, const uint32_t* d_values, const uint32_t* d_chids,
+                         uint64_t k, uint32_t* d_fills) {
+  if (!c) return WAHGPU_EINVAL;
+  if (k && (!d_values || !d_chids || !d_fills)) return fail(c, WAHGPU_EINVAL, "null buffer");
+  HIPCHK(c, hipSetDevice(c->device));
+  wah::launch_fills(d_values, d_chids, k, d_fills, c->stream);
+  LAUNCHCHK(c);
+  return WAHGPU_OK;
+}
+
+int wahgpu_fuse_fill_literals(wahgpu_ctx* c, const uint32_t* d_fills, const uint32_t* d_lits,
+                              uint64_t k, uint32_t* d_index, uint64_t* index_length) {
+  if (!c || !index_length) return WAHGPU_EINVAL;
+  if (k && (!d_fills || !d_lits || !d_index)) return fail(c, WAHGPU_EINVAL, "null buffer");
+  int rc = check_n(c, k, 0);
+  if (rc) return rc;
+  HIPCHK(c, hipSetDevice(c->device));
+  rc = arena_reserve(c, scan_scratch_bytes(k));
+  if (rc) return rc;
+  ScanScratch s = carve_scan(c, k);
+  wah::launch_fill32(s.base, s.words, 0, c->stream);
+  wah::launch_compact(s.args, d_fills, d_lits, k, 1, d_index, s.result, c->stream);
+  LAUNCHCHK(c);
+  return read_scan_result(c, s.base, index_length);
+}
+
+int wahgpu_compute_column_length(wahgpu_ctx* c, const uint32_t* d_values,
+                                 const uint32_t* d_fills, uint64_t k, uint32_t* d_keys,
+                                 uint32_t* d_offsets, uint32_t* d_lengths, uint64_t* keycnt) {
+  if (!c || !keycnt) return WAHGPU_EINVAL;
+  if (k && (!d_values || !d_fills || !d_keys || !d_offsets || !d_lengths))
+    return fail(c, WAHGPU_EINVAL, "null buffer");
+  int rc = check_n(c, k, 0);
+  if (rc) return rc;
+  HIPCHK(c, hipSetDevice(c->device));
+  rc = arena_reserve(c, scan_scratch_bytes(k) + align_up(sizeof(wahgpu_counts)));
+  if (rc) return rc;
+  ScanScratch s = carve_scan(c, k);
+  wahgpu_counts* d_counts = take<wahgpu_counts>(c, 1);
+  wah::launch_fill32(s.base, s.words, 0, c->stream);
+  wah::launch_fill32(reinterpret_cast<uint32_t*>(d_counts), sizeof(wahgpu_counts) / 4, 0,
+                     c->stream);
+  wah::launch_colen(s.args, d_values, d_fills, k, d_keys, d_offsets, d_lengths, d_counts,
+                    c->stream);
+  LAUNCHCHK(c);
+  wahgpu_counts cc;
+  uint64_t unused;
+  rc = read_scan_result(c, s.base, &unused);
+  if (rc) return rc;
+  rc = read_counts(c, d_counts, &cc);
+  if (rc) return rc;
+  *keycnt = cc.keycnt;
+  return WAHGPU_OK;
+}
+
+int wahgpu_stream_compaction(wahgpu_ctx* c, const uint32_t* d_data, const uint32_t* d_valid,
+                             uint64_t n, uint32_t* d_out, uint64_t* count) {
+  if (!c || !count) return WAHGPU_EINVAL;
+  if (n && (!d_data || !d_valid || !d_out)) return fail(c, WAHGPU_EINVAL, "null buffer");
+  int rc = check_n(c, n, 0);
+  if (rc) return rc;
+  HIPCHK(c, hipSetDevice(c->device));
+  rc = arena_reserve(c, scan_scratch_bytes(n));
+  if (rc) return rc;
+  ScanScratch s = carve_scan(c, n);
+  wah::launch_fill32(s.base, s.words, 0, c->stream);
+  wah::launch_compact(s.args, d_data, d_valid, n, 0, d_out, s.result, c->stream);
+  LAUNCHCHK(c);
+  return read_scan_result(c, s.base, count);
+}
+
+int wahgpu_exclusive_scan(wahgpu_ctx* c, const uint32_t* d_in, uint64_t n, uint32_t* d_out) {
+  if (!c) return WAHGPU_EINVAL;
+  if (n && (!d_in || !d_out)) return fail(c, WAHGPU_EINVAL, "null buffer");
+  int rc = check_n(c, n, 0);
+  if (rc) return rc;
+  HIPCHK(c, hipSetDevice(c->device));
+  rc = arena_reserve(c, scan_scratch_bytes(n));
+  if (rc) return rc;
+  ScanScratch s = carve_scan(c, n);
+  wah::launch_fill32(s.base, s.words, 0, c->stream);
+  wah::launch_scan(s.args, d_in, nullptr, n, d_out, false, c->stream);
+  LAUNCHCHK(c);
+  return WAHGPU_OK;
+}
+
+int wahgpu_segmented_inclusive_scan(wahgpu_ctx* c, const uint32_t* d_in, const uint32_t* d_heads,
+                                    uint64_t n, uint32_t* d_out) {
+  if (!c) return WAHGPU_EINVAL;
+  if (n && (!d_in || !d_heads || !d_out)) return fail(c, WAHGPU_EINVAL, "null buffer");
+  int rc = check_n(c, n, 0);
+  if (rc) return rc;
+  HIPCHK(c, hipSetDevice(c->device));
+  rc = arena_reserve(c, scan_scratch_bytes(n));
+  if (rc) return rc;
+  ScanScratch s = carve_scan(c, n);
+  wah::launch_fill32(s.base, s.words, 0, c->stream);
+  wah::launch_scan(s.args, d_in, d_heads, n, d_out, true, c->stream);
+  LAUNCHCHK(c);
+  return WAHGPU_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,23 @@
+import os
+import sys
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+if ROOT not in sys.path:
+    sys.path.insert(0, ROOT)
+
+
+def pytest_configure(config):
+    config.addinivalue_line("markers", "gpu: needs an MI355X (HIP device)")
+
+
+@pytest.fixture(scope="session")
+def ctx():
+    import torch
+
+    import indexing_amd as ia
+    assert torch.cuda.is_available(), "GPU tests need a HIP device"
+    c = ia.Context(0)
+    yield c
+    c.close()

@@ -52,7 +52,7 @@ __global__ __launch_bounds__(kBlock) void k_histogram(
   if (blockIdx.x == 0 && t < kCounters) st->tile_counter[t] = 0;
   if (blockIdx.x == 0 && t == 0) {
     counts->k = 0; counts->length = 0; counts->keycnt = 0;
-    counts->bound = 0; counts->passes = 0;
+    counts->bound = 0; counts->passes = 0; counts->fault = 0; counts->reserved = 0;
   }
   // zero look-back state used later in this build
   const uint64_t gthreads = (uint64_t)gridDim.x * kBlock;

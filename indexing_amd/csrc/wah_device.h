@@ -55,6 +55,52 @@ __device__ __forceinline__ uint64_t shfl64(uint64_t x, int src) {
   return ((uint64_t)hi << 32) | lo;
 }
 
+// Tile scheduling for the single-pass kernels.  By default a block takes the
+// next tile from a global counter, so a tile is only started after every
+// lower tile was taken by a running block (look-back always progresses).
+// WAH_STATIC_TILES=1 maps tile = blockIdx.x instead (one tile per block),
+// which relies on workgroups being dispatched in order.
+#ifndef WAH_STATIC_TILES
+#define WAH_STATIC_TILES 0
+#endif
+__device__ __forceinline__ uint32_t next_tile(uint32_t* counter, uint32_t* s_tile, uint32_t& iter) {
+#if WAH_STATIC_TILES
+  (void)counter;
+  (void)s_tile;
+  __syncthreads();
+  return iter++ == 0 ? blockIdx.x : 0xFFFFFFFFu;
+#else
+  (void)iter;
+  if (threadIdx.x == 0) *s_tile = atomicAdd(counter, 1u);
+  __syncthreads();
+  return *s_tile;
+#endif
+}
+
+// Inclusive uint32 prefix sum across the wave with DPP row shifts and row
+// broadcasts (rows of 16 lanes, then rows 1/3 and 2/3 pick up the totals).
+__device__ __forceinline__ uint32_t wave_incl_add(uint32_t x) {
+  x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x111, 0xF, 0xF, true);   // row_shr:1
+  x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x112, 0xF, 0xF, true);   // row_shr:2
+  x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x114, 0xF, 0xF, true);   // row_shr:4
+  x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x118, 0xF, 0xF, true);   // row_shr:8
+  x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x142, 0xA, 0xF, false);  // row_bcast:15
+  x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x143, 0xC, 0xF, false);  // row_bcast:31
+  return x;
+}
+__device__ __forceinline__ uint32_t readlane(uint32_t x, int lane) {
+  return (uint32_t)__builtin_amdgcn_readlane((int)x, lane);
+}
+__device__ __forceinline__ uint64_t wave_sum64(uint64_t v) {
+#pragma unroll
+  for (int d = 32; d >= 1; d >>= 1) {
+    uint32_t lo = __shfl_xor((uint32_t)v, d, kWave);
+    uint32_t hi = __shfl_xor((uint32_t)(v >> 32), d, kWave);
+    v += ((uint64_t)hi << 32) | lo;
+  }
+  return v;
+}
+
 // ---- associative operators on 62-bit look-back payloads -------------------
 // combine(a, b): a covers the earlier elements, b the later ones.
 
@@ -170,6 +216,43 @@ __device__ uint64_t lookback64(uint64_t* status, uint32_t tile, uint64_t agg, ui
     --t;
   }
   st_agent(&status[tile], ((uint64_t)kFlagPrefix << 62) | (Op::combine(acc, agg) & kMask));
+  return acc;
+}
+
+// Look-back for commutative sums, run by a whole wave: each round reads the
+// statuses of the 64 preceding tiles at once and stops at the nearest
+// inclusive prefix.  Returns the exclusive prefix in every lane.
+__device__ uint64_t lookback_wave_sum(uint64_t* status, uint32_t tile, uint64_t agg,
+                                      uint32_t* fault) {
+  constexpr uint64_t kMask = (1ull << 62) - 1;
+  const uint32_t lane = lane_id();
+  if (lane == 0)
+    st_agent(&status[tile], ((uint64_t)(tile == 0 ? kFlagPrefix : kFlagAgg) << 62) | (agg & kMask));
+  if (tile == 0) return 0;
+  uint64_t acc = 0;
+  int64_t base = (int64_t)tile - 1;
+  uint32_t spins = 0;
+  for (;;) {
+    const int64_t idx = base - (int64_t)lane;
+    const uint64_t s = idx >= 0 ? ld_agent(&status[idx]) : ((uint64_t)kFlagPrefix << 62);
+    const uint32_t f = (uint32_t)(s >> 62);
+    const uint64_t pmask = ballot(f == kFlagPrefix);
+    const uint64_t rmask = ballot(f != 0);
+    const uint64_t need = pmask ? (((pmask & (~pmask + 1)) << 1) - 1) : ~0ull;
+    if ((rmask & need) == need) {
+      acc += wave_sum64(((need >> lane) & 1u) ? (s & kMask) : 0);
+      if (pmask) break;
+      base -= kWave;
+      continue;
+    }
+    if (++spins >= kSpinLimit) {
+      if (lane == 0) atomicOr(fault, 1u);
+      break;
+    }
+    if ((spins & 1023u) == 0 && ld_agent(fault)) break;
+    __builtin_amdgcn_s_sleep(1);
+  }
+  if (lane == 0) st_agent(&status[tile], ((uint64_t)kFlagPrefix << 62) | ((acc + agg) & kMask));
   return acc;
 }
 

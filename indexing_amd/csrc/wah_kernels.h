@@ -11,13 +11,28 @@ typedef wahgpu_counts wahgpu_counts_dev;
 
 namespace wah {
 
-constexpr int kSortItems = 16;                  // keys per thread in a radix pass
-constexpr int kSortTile = 256 * kSortItems;     // 4096 keys per tile
-constexpr int kTailItems = 8;
-constexpr int kTailTile = 256 * kTailItems;     // 2048 sorted pairs per tail tile
+// Tuning knobs (compile-time; defaults are the measured best on MI355X).
+#ifndef WAH_SORT_THREADS
+#define WAH_SORT_THREADS 512
+#endif
+#ifndef WAH_SORT_ITEMS
+#define WAH_SORT_ITEMS 16
+#endif
+#ifndef WAH_TAIL_THREADS
+#define WAH_TAIL_THREADS 512
+#endif
+#ifndef WAH_TAIL_ITEMS
+#define WAH_TAIL_ITEMS 8
+#endif
+constexpr int kSortThreads = WAH_SORT_THREADS;  // threads of a radix-pass block
+constexpr int kSortItems = WAH_SORT_ITEMS;      // keys per thread in a radix pass
+constexpr int kSortTile = kSortThreads * kSortItems;
+constexpr int kTailThreads = WAH_TAIL_THREADS;
+constexpr int kTailItems = WAH_TAIL_ITEMS;
+constexpr int kTailTile = kTailThreads * kTailItems;  // sorted pairs per tail tile
 constexpr int kScanItems = 8;
 constexpr int kScanTile = 256 * kScanItems;
-constexpr uint32_t kHistMaxBlocks = 1024;
+constexpr uint32_t kHistMaxBlocks = 2048;
 constexpr int kCounters = 8;
 constexpr int kCounterTail = 4;
 
@@ -35,13 +50,18 @@ struct PassInfo {
 };
 
 struct SortState {
+  // --- zeroed before every build (kSortHeaderWords words) ---
+  uint32_t hist[4][256];          // digit histograms of the column
+  uint32_t done;                  // histogram blocks finished
+  uint32_t maxval;
+  uint32_t tile_counter[kCounters];
+  // --- written by the last histogram block ---
   uint32_t digit_offset[4][256];  // exclusive prefix of the digit histogram
   PassInfo pass[4];
   uint32_t num_active;
   uint32_t final_buf;             // buffer holding the sorted pairs
-  uint32_t maxval;
-  uint32_t tile_counter[kCounters];
 };
+constexpr uint64_t kSortHeaderWords = 4 * 256 + 2 + kCounters;
 
 struct BuildArgs {
   const uint32_t* values;
@@ -49,14 +69,12 @@ struct BuildArgs {
   uint32_t rid_base;
   uint32_t *keysA, *ridsA, *keysB, *ridsB;
   SortState* st;
-  uint32_t* part_hist;     // kHistMaxBlocks x 1024
-  uint32_t* part_max;      // kHistMaxBlocks
   uint32_t* sort_status;   // sort_status_words(n)
   uint64_t* tail_status;   // tail_status_words(n)
   uint32_t *out_keys, *out_offsets, *out_lengths, *out_words;
   uint32_t *sorted_keys, *sorted_rids;  // stage API sort output
   wahgpu_counts_dev* counts;
-  uint32_t max_resident;   // persistent grid size
+  uint32_t max_resident;   // resident 256-thread blocks on the device
 };
 
 struct ScanArgs {

@@ -1,10 +1,11 @@
 // wah_kernels.hip -- gfx950 kernels of the WAH bitmap-index build.
 //
 // Fused build (what wahgpu_build runs), for a column of n uint32 values:
-//   k_histogram   one read of the values: per-block histograms of all four
-//                 8-bit digits, the maximum, and zeroing of look-back state
-//   k_hist_scan   digit offsets per pass; passes whose digit is the same for
-//                 every value are switched off (device-side, no host sync)
+//   k_fill32      zeroes the sort header (histogram, counters)
+//   k_histogram   one read of the values: histograms of all four 8-bit
+//                 digits and the maximum, zeroing of look-back state; the
+//                 last block scans the digit offsets and switches off passes
+//                 whose digit is the same for every value (no host sync)
 //   k_onesweep x4 stable LSD radix pass per active digit: wave64 ballot
 //                 ranking, LDS staging, decoupled look-back over tiles
 //   k_tail        sorted (value, rid) -> chunk ids, OR-merged literals, fills,
@@ -22,17 +23,23 @@
 
 namespace wah {
 
+__global__ __launch_bounds__(kBlock) void k_fill32(uint32_t* __restrict__ p, uint64_t words,
+                                                   uint32_t v) {
+  for (uint64_t i = (uint64_t)blockIdx.x * kBlock + threadIdx.x; i < words;
+       i += (uint64_t)gridDim.x * kBlock)
+    p[i] = v;
+}
+
 // ---------------------------------------------------------------------------
 // Histogram of all four 8-bit digits (reads the values once).
-// Zeroes the first status array of the sort, the tail status and counters.
 // ---------------------------------------------------------------------------
 __device__ __forceinline__ void hist_add(uint32_t* h, uint32_t bin, bool valid) {
   // A wave whose valid lanes all share the bin adds once (the high digits of
   // small-range columns are constant); otherwise one LDS atomic per lane.
-  uint32_t first = __builtin_amdgcn_readfirstlane(bin);
-  uint64_t differ = ballot(valid && bin != first);
+  const uint32_t first = __builtin_amdgcn_readfirstlane(bin);
+  const uint64_t differ = ballot(valid && bin != first);
   if (differ == 0) {
-    uint64_t cnt = ballot(valid);
+    const uint64_t cnt = ballot(valid);
     if (lane_id() == 0 && cnt) atomicAdd(&h[first], (uint32_t)__popcll(cnt));
   } else if (valid) {
     atomicAdd(&h[bin], 1u);
@@ -40,16 +47,17 @@ __device__ __forceinline__ void hist_add(uint32_t* h, uint32_t bin, bool valid) 
 }
 
 __global__ __launch_bounds__(kBlock) void k_histogram(
-    const uint32_t* __restrict__ values, uint64_t n, int vec4,
-    uint32_t* __restrict__ part, uint32_t* __restrict__ part_max,
-    SortState* __restrict__ st, uint32_t* __restrict__ status0, uint64_t status0_words,
+    const uint32_t* __restrict__ values, uint64_t n, int vec4, SortState* __restrict__ st,
+    uint32_t* __restrict__ status0, uint64_t status0_words,
     uint64_t* __restrict__ tail_status, uint64_t tail_words,
     wahgpu_counts_dev* __restrict__ counts) {
   __shared__ uint32_t h[4 * 256];
   __shared__ uint32_t s_max[kWaves];
+  __shared__ uint64_t s_scan[kWaves];
+  __shared__ int s_trivial[4];
+  __shared__ int s_last;
   const int t = threadIdx.x;
   for (int i = t; i < 1024; i += kBlock) h[i] = 0;
-  if (blockIdx.x == 0 && t < kCounters) st->tile_counter[t] = 0;
   if (blockIdx.x == 0 && t == 0) {
     counts->k = 0; counts->length = 0; counts->keycnt = 0;
     counts->bound = 0; counts->passes = 0; counts->fault = 0; counts->reserved = 0;
@@ -62,75 +70,70 @@ __global__ __launch_bounds__(kBlock) void k_histogram(
   __syncthreads();
 
   uint32_t vmax = 0;
-  // Contiguous range per block, a multiple of 1024 values so that every
-  // block but the last starts and ends on a 16-byte boundary.
-  const uint64_t per = ((n + gridDim.x - 1) / gridDim.x + 1023) / 1024 * 1024;
-  uint64_t lo = (uint64_t)blockIdx.x * per;
-  if (lo > n) lo = n;
-  const uint64_t hi = lo + per < n ? lo + per : n;
-  uint64_t scalar_lo = lo;
+  // Grid-stride over chunks of 4096 values, four 16-byte loads per thread.
+  constexpr uint64_t kChunk = 4 * 4 * kBlock;
+  uint64_t scalar_lo = 0;
   if (vec4) {
     const uint4* v4 = reinterpret_cast<const uint4*>(values);
-    const uint64_t lo4 = lo / 4, hi4 = hi / 4;
-    for (uint64_t base = lo4; base < hi4; base += kBlock) {
-      uint64_t i = base + t;
-      bool valid = i < hi4;
-      uint4 q = valid ? v4[i] : make_uint4(0, 0, 0, 0);
-      uint32_t e[4] = {q.x, q.y, q.z, q.w};
+    const uint64_t full = n / kChunk;
+    for (uint64_t c = blockIdx.x; c < full; c += gridDim.x) {
+      uint4 q[4];
 #pragma unroll
-      for (int j = 0; j < 4; ++j) {
-        vmax = valid && e[j] > vmax ? e[j] : vmax;
+      for (int u = 0; u < 4; ++u) q[u] = v4[c * (kChunk / 4) + u * kBlock + t];
 #pragma unroll
-        for (int p = 0; p < 4; ++p) hist_add(h, p * 256 + ((e[j] >> (8 * p)) & 255u), valid);
+      for (int u = 0; u < 4; ++u) {
+        const uint32_t e[4] = {q[u].x, q[u].y, q[u].z, q[u].w};
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+          vmax = e[j] > vmax ? e[j] : vmax;
+#pragma unroll
+          for (int p = 0; p < 4; ++p) hist_add(h, p * 256 + ((e[j] >> (8 * p)) & 255u), true);
+        }
       }
     }
-    scalar_lo = hi4 * 4;
+    scalar_lo = full * kChunk;
   }
-  for (uint64_t base = scalar_lo; base < hi; base += kBlock) {
-    uint64_t i = base + t;
-    bool valid = i < hi;
-    uint32_t v = valid ? values[i] : 0;
+  // remainder (or unaligned input): one value per thread per step
+  for (uint64_t b = scalar_lo + (uint64_t)blockIdx.x * kBlock; b < n;
+       b += (uint64_t)gridDim.x * kBlock) {
+    const uint64_t i = b + t;
+    const bool valid = i < n;
+    const uint32_t v = valid ? values[i] : 0;
     vmax = valid && v > vmax ? v : vmax;
 #pragma unroll
     for (int p = 0; p < 4; ++p) hist_add(h, p * 256 + ((v >> (8 * p)) & 255u), valid);
   }
-  // block max
   for (int d = 32; d >= 1; d >>= 1) {
-    uint32_t o = __shfl_xor(vmax, d, kWave);
+    const uint32_t o = __shfl_xor(vmax, d, kWave);
     vmax = o > vmax ? o : vmax;
   }
   if ((t % kWave) == 0) s_max[t / kWave] = vmax;
   __syncthreads();
-  for (int i = t; i < 1024; i += kBlock) part[(uint64_t)blockIdx.x * 1024 + i] = h[i];
+  // fold into the column histogram (st->hist was zeroed by the previous launch)
+  for (int i = t; i < 1024; i += kBlock)
+    if (h[i]) atomicAdd(&st->hist[0][0] + i, h[i]);
   if (t == 0) {
     uint32_t m = 0;
     for (int i = 0; i < kWaves; ++i) m = s_max[i] > m ? s_max[i] : m;
-    part_max[blockIdx.x] = m;
+    atomicMax(&st->maxval, m);
   }
-}
-
-// One block: sum the per-block histograms, scan digit offsets, choose passes.
-__global__ __launch_bounds__(kBlock) void k_hist_scan(
-    const uint32_t* __restrict__ part, const uint32_t* __restrict__ part_max,
-    uint32_t nblocks, uint64_t n, SortState* __restrict__ st,
-    wahgpu_counts_dev* __restrict__ counts) {
-  __shared__ uint64_t s_scan[kWaves];
-  __shared__ int s_trivial[4];
-  const int t = threadIdx.x;
+  __threadfence();
+  __syncthreads();
+  if (t == 0) s_last = atomicAdd(&st->done, 1u) == gridDim.x - 1;
   if (t < 4) s_trivial[t] = 0;
   __syncthreads();
+  if (!s_last) return;
+  // The last block: digit offsets and the pass plan.
+  __threadfence();
   for (int p = 0; p < 4; ++p) {
-    uint32_t c = 0;
-    for (uint32_t b = 0; b < nblocks; ++b) c += part[(uint64_t)b * 1024 + p * 256 + t];
+    const uint32_t c = ld_agent(&st->hist[p][t]);
     uint64_t total;
-    uint64_t ex = block_exclusive<OpAdd>(c, s_scan, &total);
+    const uint64_t ex = block_exclusive<OpAdd>(c, s_scan, &total);
     st->digit_offset[p][t] = (uint32_t)ex;
     if ((uint64_t)c == n) s_trivial[p] = 1;  // every value has digit t
   }
   __syncthreads();
   if (t == 0) {
-    uint32_t m = 0;
-    for (uint32_t b = 0; b < nblocks; ++b) m = part_max[b] > m ? part_max[b] : m;
     uint32_t a = 0;
     for (int p = 0; p < 4; ++p) {
       PassInfo pi;
@@ -145,33 +148,35 @@ __global__ __launch_bounds__(kBlock) void k_hist_scan(
     }
     st->num_active = a;
     st->final_buf = (a == 0) ? kBufValues : ((a - 1) % 2 == 0 ? kBufA : kBufB);
-    st->maxval = m;
-    counts->bound = m;
+    counts->bound = ld_agent(&st->maxval);
     counts->passes = a;
   }
 }
 
 // ---------------------------------------------------------------------------
 // One stable LSD radix pass over 8-bit digit p (Onesweep-style).
-// Tile = 256 threads x ITEMS keys; wave w ranks keys [w*64*ITEMS, +64*ITEMS)
-// of the tile in (item, lane) order, which is the input order, so equal
-// digits keep their relative order (stability).
+// Tile = THREADS x ITEMS keys; wave w ranks keys [w*64*ITEMS, +64*ITEMS) of
+// the tile in (item, lane) order, which is the input order, so equal digits
+// keep their relative order (stability).  Keys and then rids are staged in
+// one LDS buffer in digit order and written out in runs per digit.
 // ---------------------------------------------------------------------------
-template <int ITEMS>
-__global__ __launch_bounds__(kBlock) void k_onesweep(
+template <int THREADS, int ITEMS>
+__global__ __launch_bounds__(THREADS, 2) void k_onesweep(
     int p, const uint32_t* __restrict__ values, uint32_t* keysA, uint32_t* ridsA,
     uint32_t* keysB, uint32_t* ridsB, uint64_t n, uint32_t rid_base,
     SortState* __restrict__ st, uint32_t* status_base, uint64_t status_stride,
     uint32_t num_tiles, uint32_t* fault) {
-  constexpr int TILE = kBlock * ITEMS;
+  constexpr int TILE = THREADS * ITEMS;
+  constexpr int WAVES = THREADS / kWave;
+  static_assert(THREADS >= 256 && THREADS % 256 == 0, "one thread per digit");
   const PassInfo info = st->pass[p];
   if (!info.active) return;
-  __shared__ uint32_t s_keys[TILE];
-  __shared__ uint32_t s_rids[TILE];
-  __shared__ uint32_t s_wcnt[kWaves][256];
+  __shared__ uint32_t s_stage[TILE];
+  __shared__ uint8_t s_digit[TILE];
+  __shared__ uint32_t s_wcnt[WAVES][256];
   __shared__ uint32_t s_excl[256];
   __shared__ uint32_t s_glob[256];
-  __shared__ uint64_t s_scan[kWaves];
+  __shared__ uint32_t s_dsum[4];
   __shared__ uint32_t s_tile;
 
   const bool implicit = info.src == kBufValues;
@@ -183,24 +188,22 @@ __global__ __launch_bounds__(kBlock) void k_onesweep(
   uint32_t* status_next = status_base + (uint64_t)(1u - info.status_sel) * status_stride;
   const uint32_t shift = info.shift;
   const int t = threadIdx.x, w = t / kWave, l = t % kWave;
-  const uint32_t goff = st->digit_offset[p][t];
+  const uint32_t goff = t < 256 ? st->digit_offset[p][t] : 0u;
 
+  uint32_t iter = 0;
   for (;;) {
-    if (t == 0) s_tile = atomicAdd(&st->tile_counter[p], 1u);
-#pragma unroll
-    for (int i = 0; i < kWaves; ++i) s_wcnt[i][t] = 0;
-    __syncthreads();
-    const uint32_t tile = s_tile;
+    for (int i = t; i < WAVES * 256; i += THREADS) (&s_wcnt[0][0])[i] = 0;
+    const uint32_t tile = next_tile(&st->tile_counter[p], &s_tile, iter);
     if (tile >= num_tiles) break;
     const uint64_t base = (uint64_t)tile * TILE;
-    status_next[(uint64_t)tile * 256 + t] = 0;  // the next active pass's state
+    if (t < 256) status_next[(uint64_t)tile * 256 + t] = 0;  // next active pass's state
 
     uint32_t key[ITEMS], rid[ITEMS], rank[ITEMS];
     const uint64_t wbase = base + (uint64_t)w * kWave * ITEMS + l;
 #pragma unroll
     for (int i = 0; i < ITEMS; ++i) {
-      uint64_t idx = wbase + (uint64_t)i * kWave;
-      bool valid = idx < n;
+      const uint64_t idx = wbase + (uint64_t)i * kWave;
+      const bool valid = idx < n;
       key[i] = valid ? src_keys[idx] : 0u;
       rid[i] = implicit ? rid_base + (uint32_t)idx : (valid ? src_rids[idx] : 0u);
     }
@@ -208,70 +211,97 @@ __global__ __launch_bounds__(kBlock) void k_onesweep(
     // below this lane + keys of that digit the wave ranked before.
 #pragma unroll
     for (int i = 0; i < ITEMS; ++i) {
-      uint64_t idx = wbase + (uint64_t)i * kWave;
+      const uint64_t idx = wbase + (uint64_t)i * kWave;
       if (idx < n) {
         const uint32_t d = (key[i] >> shift) & 255u;
         uint64_t peers = ballot(true);
 #pragma unroll
         for (int b = 0; b < 8; ++b) {
-          bool bit = (d >> b) & 1u;
-          uint64_t bb = ballot(bit);
+          const bool bit = (d >> b) & 1u;
+          const uint64_t bb = ballot(bit);
           peers &= bit ? bb : ~bb;
         }
-        uint32_t cnt = (uint32_t)__popcll(peers);
-        uint32_t below = mbcnt(peers);
-        uint32_t prior = s_wcnt[w][d];
+        const uint32_t cnt = (uint32_t)__popcll(peers);
+        const uint32_t below = mbcnt(peers);
+        const uint32_t prior = s_wcnt[w][d];
         rank[i] = prior + below;
         if (below == cnt - 1) s_wcnt[w][d] = prior + cnt;
       }
     }
     __syncthreads();
-    // Thread t owns digit t: wave prefixes, tile total, look-back.
-    uint32_t c0 = s_wcnt[0][t], c1 = s_wcnt[1][t], c2 = s_wcnt[2][t], c3 = s_wcnt[3][t];
-    const uint32_t total = c0 + c1 + c2 + c3;
-    s_wcnt[0][t] = 0; s_wcnt[1][t] = c0; s_wcnt[2][t] = c0 + c1; s_wcnt[3][t] = c0 + c1 + c2;
-    uint32_t* my = &status[(uint64_t)tile * 256 + t];
-    st_agent(my, ((tile == 0 ? kFlagPrefix : kFlagAgg) << 30) | total);
-    uint64_t dummy;
-    const uint32_t excl = (uint32_t)block_exclusive<OpAdd>(total, s_scan, &dummy);
-    uint32_t prev = 0;
-    if (tile > 0) {
-      int64_t tt = (int64_t)tile - 1;
-      while (tt >= 0) {
-        uint32_t s = wait_status(&status[(uint64_t)tt * 256 + t], fault);
-        prev += s & 0x3FFFFFFFu;
-        if ((s >> 30) != kFlagAgg) break;  // prefix found (or gave up)
-        --tt;
+    // Threads 0..255 own one digit each: wave prefixes, tile total,
+    // exclusive scan over digits, look-back.
+    if (t < 256) {
+      uint32_t total = 0;
+#pragma unroll
+      for (int i = 0; i < WAVES; ++i) {
+        const uint32_t c = s_wcnt[i][t];
+        s_wcnt[i][t] = total;
+        total += c;
       }
-      st_agent(my, (kFlagPrefix << 30) | (prev + total));
+      uint32_t* my = &status[(uint64_t)tile * 256 + t];
+      st_agent(my, ((tile == 0 ? kFlagPrefix : kFlagAgg) << 30) | total);
+      const uint32_t inc = wave_incl_add(total);
+      if (l == kWave - 1) s_dsum[w] = inc;
+      s_excl[t] = inc - total;  // within this wave's 64 digits
+      uint32_t prev = 0;
+      if (tile > 0) {
+        int64_t tt = (int64_t)tile - 1;
+        while (tt >= 0) {
+          const uint32_t s = wait_status(&status[(uint64_t)tt * 256 + t], fault);
+          prev += s & 0x3FFFFFFFu;
+          if ((s >> 30) != kFlagAgg) break;  // prefix found (or gave up)
+          --tt;
+        }
+        st_agent(my, (kFlagPrefix << 30) | (prev + total));
+      }
+      s_glob[t] = goff + prev;  // the tile-local offset is subtracted below
     }
-    s_excl[t] = excl;
-    s_glob[t] = goff + prev - excl;  // global index = s_glob[d] + local index
+    __syncthreads();
+    if (t < 256) {
+      uint32_t before = 0;
+#pragma unroll
+      for (int i = 0; i < 4; ++i) before += i < w ? s_dsum[i] : 0u;
+      const uint32_t ex = s_excl[t] + before;
+      s_excl[t] = ex;
+      s_glob[t] -= ex;  // global index = s_glob[d] + local index
+    }
     __syncthreads();
 #pragma unroll
     for (int i = 0; i < ITEMS; ++i) {
-      uint64_t idx = wbase + (uint64_t)i * kWave;
+      const uint64_t idx = wbase + (uint64_t)i * kWave;
       if (idx < n) {
         const uint32_t d = (key[i] >> shift) & 255u;
-        uint32_t pos = s_excl[d] + s_wcnt[w][d] + rank[i];
-        s_keys[pos] = key[i];
-        s_rids[pos] = rid[i];
+        const uint32_t pos = s_excl[d] + s_wcnt[w][d] + rank[i];
+        s_stage[pos] = key[i];
+        s_digit[pos] = (uint8_t)d;
+        rank[i] = pos;  // keep the LDS slot for the rids
       }
     }
     __syncthreads();
     const uint32_t tile_n = (n - base) < (uint64_t)TILE ? (uint32_t)(n - base) : (uint32_t)TILE;
 #pragma unroll
     for (int j = 0; j < ITEMS; ++j) {
-      uint32_t pos = (uint32_t)j * kBlock + t;
-      if (pos < tile_n) {
-        uint32_t k = s_keys[pos];
-        uint32_t g = s_glob[(k >> shift) & 255u] + pos;
-        if (g < n) {
-          dst_keys[g] = k;
-          dst_rids[g] = s_rids[pos];
-        } else {
-          atomicOr(fault, 2u);
-        }
+      const uint32_t q = (uint32_t)j * THREADS + t;
+      if (q < tile_n) {
+        const uint32_t g = s_glob[s_digit[q]] + q;
+        if (g < n) dst_keys[g] = s_stage[q];
+        else atomicOr(fault, 2u);
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < ITEMS; ++i) {
+      const uint64_t idx = wbase + (uint64_t)i * kWave;
+      if (idx < n) s_stage[rank[i]] = rid[i];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < ITEMS; ++j) {
+      const uint32_t q = (uint32_t)j * THREADS + t;
+      if (q < tile_n) {
+        const uint32_t g = s_glob[s_digit[q]] + q;
+        if (g < n) dst_rids[g] = s_stage[q];
       }
     }
     __syncthreads();
@@ -288,7 +318,7 @@ __device__ __forceinline__ SortedView sorted_view(const SortState* st, const uin
                                                   const uint32_t* keysA, const uint32_t* ridsA,
                                                   const uint32_t* keysB, const uint32_t* ridsB) {
   SortedView v;
-  uint32_t f = st->final_buf;
+  const uint32_t f = st->final_buf;
   v.implicit = f == kBufValues;
   v.keys = f == kBufValues ? values : (f == kBufA ? keysA : keysB);
   v.rids = f == kBufA ? ridsA : ridsB;
@@ -296,63 +326,101 @@ __device__ __forceinline__ SortedView sorted_view(const SortState* st, const uin
 }
 
 // ---------------------------------------------------------------------------
-// Fused tail over the sorted (value, rid) pairs.  Thread t of a tile owns
-// ITEMS consecutive elements.  Per element: chid = rid/31; an entry starts
-// where (value, chid) changes and ORs the literal bits of its run (at most 31
-// elements); a key starts where value changes.  fill = chid for a key start,
-// else chid - chid_prev - 1; an entry emits [fill if nonzero, literal].
-// Word and key positions come from a look-back scan of (keys, words).
+// Fused tail over the sorted (value, rid) pairs.  A tile of THREADS x ITEMS
+// pairs is staged in LDS; wave w owns pairs [w*64*ITEMS, +64*ITEMS) in (item,
+// lane) order, so lanes read consecutive LDS words.  Per pair: chid = rid/31;
+// an entry starts where (value, chid) changes and ORs the literal bits of its
+// run (at most 31 pairs, read ahead in LDS); a key starts where value
+// changes.  fill = chid for a key start, else chid - chid_prev - 1; an entry
+// emits [fill if nonzero, literal].  Word and key positions: DPP wave scans of
+// the packed (keys << 16 | words) counts, then a wave-parallel look-back
+// across tiles.  Words are staged in LDS and stored contiguously.
 // ---------------------------------------------------------------------------
-template <int ITEMS>
-__global__ __launch_bounds__(kBlock) void k_tail(
+template <int THREADS, int ITEMS>
+__global__ __launch_bounds__(THREADS) void k_tail(
     const SortState* __restrict__ st, const uint32_t* values, const uint32_t* keysA,
     const uint32_t* ridsA, const uint32_t* keysB, const uint32_t* ridsB, uint64_t n,
     uint32_t rid_base, uint32_t* __restrict__ out_keys, uint32_t* __restrict__ out_offsets,
     uint32_t* __restrict__ out_words, uint64_t* status, uint32_t num_tiles,
     uint32_t* tile_counter, wahgpu_counts_dev* __restrict__ counts) {
-  constexpr int TILE = kBlock * ITEMS;
-  constexpr int HALO = 32;
-  __shared__ uint32_t s_k[TILE + HALO];   // s_k[j] = element base - 1 + j
-  __shared__ uint32_t s_r[TILE + HALO];
-  __shared__ uint32_t s_words[2 * TILE];
-  __shared__ uint64_t s_scan[kWaves];
+  constexpr int TILE = THREADS * ITEMS;
+  constexpr int WAVES = THREADS / kWave;
+  constexpr int SUB = kWave * ITEMS;     // pairs per wave
+  constexpr int PAD = 4;                 // s_k[PAD - 1] holds pair base - 1
+  constexpr int HALO = 32;               // pairs read past the tile by runs
+  constexpr int SPAN = TILE + PAD + HALO;
+  static_assert(TILE % 4 == 0 && TILE <= 32768, "word counts fit 16 bits");
+  __shared__ __align__(16) uint32_t s_buf[2 * SPAN];
+  __shared__ uint32_t s_wtot[WAVES];
   __shared__ uint64_t s_prefix;
   __shared__ uint32_t s_tile;
+  uint32_t* s_k = s_buf;
+  uint32_t* s_r = s_buf + SPAN;
+  uint32_t* s_words = s_buf;  // reused once the pairs are consumed (2 * TILE words)
   const SortedView sv = sorted_view(st, values, keysA, ridsA, keysB, ridsB);
-  const int t = threadIdx.x;
+  const int t = threadIdx.x, w = t / kWave, l = t % kWave;
+  uint64_t entries = 0;
 
+  uint32_t iter = 0;
   for (;;) {
-    if (t == 0) s_tile = atomicAdd(tile_counter, 1u);
-    __syncthreads();
-    const uint32_t tile = s_tile;
+    const uint32_t tile = next_tile(tile_counter, &s_tile, iter);
     if (tile >= num_tiles) break;
     const uint64_t base = (uint64_t)tile * TILE;
-    for (int j = t; j < TILE + HALO; j += kBlock) {
-      int64_t g = (int64_t)base - 1 + j;
-      if (g >= 0 && (uint64_t)g < n) {
-        s_k[j] = sv.keys[g];
-        s_r[j] = sv.implicit ? rid_base + (uint32_t)g : sv.rids[g];
+    // stage pairs [base - 1, base + TILE + HALO - 1) with 16-byte loads
+    if (base + TILE <= n) {
+      const uint4* k4 = reinterpret_cast<const uint4*>(sv.keys + base);
+      const uint4* r4 = reinterpret_cast<const uint4*>(sv.rids + base);
+      for (int j = t; j < TILE / 4; j += THREADS) {
+        reinterpret_cast<uint4*>(s_k + PAD)[j] = k4[j];
+        uint4 rr;
+        if (sv.implicit) {
+          const uint32_t r0 = rid_base + (uint32_t)(base + 4 * (uint64_t)j);
+          rr = make_uint4(r0, r0 + 1, r0 + 2, r0 + 3);
+        } else {
+          rr = r4[j];
+        }
+        reinterpret_cast<uint4*>(s_r + PAD)[j] = rr;
       }
+    } else {
+      for (int j = t; j < TILE; j += THREADS) {
+        const uint64_t g = base + j;
+        if (g < n) {
+          s_k[PAD + j] = sv.keys[g];
+          s_r[PAD + j] = sv.implicit ? rid_base + (uint32_t)g : sv.rids[g];
+        }
+      }
+    }
+    if (t < HALO) {
+      const uint64_t g = base + TILE + t;
+      if (g < n) {
+        s_k[PAD + TILE + t] = sv.keys[g];
+        s_r[PAD + TILE + t] = sv.implicit ? rid_base + (uint32_t)g : sv.rids[g];
+      }
+    }
+    if (t == THREADS - 1 && base > 0) {
+      s_k[PAD - 1] = sv.keys[base - 1];
+      s_r[PAD - 1] = sv.implicit ? rid_base + (uint32_t)(base - 1) : sv.rids[base - 1];
     }
     __syncthreads();
 
-    uint32_t fill[ITEMS], lit[ITEMS];
+    uint32_t fill[ITEMS], lit[ITEMS], key[ITEMS], ex[ITEMS];
     uint32_t khmask = 0;
-    uint32_t W = 0, KH = 0, EH = 0;
-    const int j0 = t * ITEMS;
+    uint32_t carry = 0;  // (keys << 16 | words) of this wave's earlier rows
 #pragma unroll
     for (int i = 0; i < ITEMS; ++i) {
-      const uint64_t g = base + j0 + i;
+      const int j = w * SUB + i * kWave + l;
+      const uint64_t g = base + j;
+      const int s = PAD + j;
       fill[i] = 0;
       lit[i] = 0;
+      key[i] = 0;
+      uint32_t p = 0;
       if (g < n) {
-        const int s = 1 + j0 + i;
         const uint32_t v = s_k[s], r = s_r[s], c = r / 31u;
         const bool first = g == 0;
         const uint32_t cp = first ? 0u : s_r[s - 1] / 31u;
         const bool kh = first || v != s_k[s - 1];
-        const bool eh = kh || c != cp;
-        if (eh) {
+        if (kh || c != cp) {
           uint32_t x = 0x80000000u | (1u << (r % 31u));
           for (int m = 1; m < 31; ++m) {
             if (g + m >= n) break;
@@ -362,53 +430,69 @@ __global__ __launch_bounds__(kBlock) void k_tail(
           }
           lit[i] = x;
           fill[i] = kh ? c : c - cp - 1u;
-          W += 1u + (fill[i] != 0u);
-          ++EH;
+          p = 1u + (fill[i] != 0u);
+          ++entries;
         }
-        if (kh) { khmask |= 1u << i; ++KH; }
+        if (kh) {
+          key[i] = v;
+          khmask |= 1u << i;
+          p |= 1u << 16;
+        }
       }
+      const uint32_t inc = wave_incl_add(p);
+      ex[i] = carry + inc - p;
+      carry += readlane(inc, kWave - 1);
     }
-    uint64_t tile_agg;
-    const uint64_t ex = block_exclusive<OpAdd>(((uint64_t)KH << 31) | W, s_scan, &tile_agg);
-    if (t == 0) s_prefix = lookback64<OpAdd>(status, tile, tile_agg, &counts->fault);
-    // entry count for k
-    uint64_t ehs = EH;
-    for (int d = 32; d >= 1; d >>= 1) ehs += __shfl_xor((uint32_t)ehs, d, kWave);
-    if ((t % kWave) == 0 && ehs) atomicAdd((unsigned long long*)&counts->k, (unsigned long long)(uint32_t)ehs);
+    if (l == 0) s_wtot[w] = carry;
     __syncthreads();
+    uint32_t woff = 0, tot = 0;
+#pragma unroll
+    for (int i = 0; i < WAVES; ++i) {
+      woff += i < w ? s_wtot[i] : 0u;
+      tot += s_wtot[i];
+    }
+    const uint64_t tile_agg = ((uint64_t)(tot >> 16) << 31) | (tot & 0xFFFFu);
+    if (w == 0) {
+      const uint64_t pre = lookback_wave_sum(status, tile, tile_agg, &counts->fault);
+      if (l == 0) s_prefix = pre;
+    }
+    __syncthreads();  // every read of the staged pairs is done
     const uint64_t pre = s_prefix;
     const uint32_t Wpre = (uint32_t)(pre & 0x7FFFFFFFu), Kpre = (uint32_t)(pre >> 31);
-    uint32_t pos = (uint32_t)(ex & 0x7FFFFFFFu);
-    uint32_t kidx = Kpre + (uint32_t)(ex >> 31);
 #pragma unroll
     for (int i = 0; i < ITEMS; ++i) {
-      if (khmask & (1u << i)) {
+      const uint32_t e = woff + ex[i];
+      uint32_t pos = e & 0xFFFFu;
+      if ((khmask >> i) & 1u) {
+        const uint64_t kidx = (uint64_t)Kpre + (e >> 16);
         if (kidx < n) {
-          out_keys[kidx] = s_k[1 + j0 + i];
+          out_keys[kidx] = key[i];
           out_offsets[kidx] = Wpre + pos;
         } else {
           atomicOr(&counts->fault, 2u);
         }
-        ++kidx;
       }
       if (lit[i]) {
         if (fill[i]) s_words[pos++] = fill[i];
-        s_words[pos++] = lit[i];
+        s_words[pos] = lit[i];
       }
     }
     __syncthreads();
-    const uint32_t Wt = (uint32_t)(tile_agg & 0x7FFFFFFFu);
-    for (uint32_t j = t; j < Wt; j += kBlock) {
-      uint64_t o = (uint64_t)Wpre + j;
+    const uint32_t Wt = tot & 0xFFFFu;
+    for (uint32_t j = t; j < Wt; j += THREADS) {
+      const uint64_t o = (uint64_t)Wpre + j;
       if (o < 2 * n) out_words[o] = s_words[j];
       else atomicOr(&counts->fault, 2u);
     }
     if (tile == num_tiles - 1 && t == 0) {
       counts->length = (uint64_t)Wpre + Wt;
-      counts->keycnt = (uint64_t)Kpre + (uint32_t)(tile_agg >> 31);
+      counts->keycnt = (uint64_t)Kpre + (tot >> 16);
     }
     __syncthreads();
   }
+  // one update of k per wave
+  entries = wave_sum64(entries);
+  if (l == 0 && entries) atomicAdd((unsigned long long*)&counts->k, (unsigned long long)entries);
 }
 
 // lengths[i] = offsets[i+1] - offsets[i]; the last key runs to L.
@@ -419,7 +503,7 @@ __global__ __launch_bounds__(kBlock) void k_lengths(const uint32_t* __restrict__
   const uint64_t K = counts->keycnt < cap ? counts->keycnt : cap, L = counts->length;
   for (uint64_t i = (uint64_t)blockIdx.x * kBlock + threadIdx.x; i < K;
        i += (uint64_t)gridDim.x * kBlock) {
-    uint32_t end = (i + 1 < K) ? offsets[i + 1] : (uint32_t)L;
+    const uint32_t end = (i + 1 < K) ? offsets[i + 1] : (uint32_t)L;
     lengths[i] = end - offsets[i];
   }
 }
@@ -427,13 +511,6 @@ __global__ __launch_bounds__(kBlock) void k_lengths(const uint32_t* __restrict__
 // ---------------------------------------------------------------------------
 // Staged kernels (one per reference stage) and primitives.
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(kBlock) void k_fill32(uint32_t* __restrict__ p, uint64_t words,
-                                                   uint32_t v) {
-  for (uint64_t i = (uint64_t)blockIdx.x * kBlock + threadIdx.x; i < words;
-       i += (uint64_t)gridDim.x * kBlock)
-    p[i] = v;
-}
-
 // Copy the sorted result out of the pass buffers (stage API output).
 __global__ __launch_bounds__(kBlock) void k_sort_finalize(
     const SortState* __restrict__ st, const uint32_t* values, const uint32_t* keysA,
@@ -705,51 +782,49 @@ __global__ __launch_bounds__(kBlock) void k_scan(
 // ---------------------------------------------------------------------------
 // Launchers
 // ---------------------------------------------------------------------------
-template <int ITEMS>
 static void launch_onesweep(const BuildArgs& a, int p, hipStream_t s) {
-  constexpr int TILE = kBlock * ITEMS;
-  uint32_t tiles = (uint32_t)((a.n + TILE - 1) / TILE);
-  uint32_t grid = tiles < a.max_resident ? tiles : a.max_resident;
+  const uint32_t tiles = (uint32_t)((a.n + kSortTile - 1) / kSortTile);
+  const uint32_t cap = WAH_STATIC_TILES ? tiles : a.max_resident * 256 / kSortThreads;
+  uint32_t grid = tiles < cap ? tiles : cap;
   if (grid == 0) grid = 1;
-  hipLaunchKernelGGL(k_onesweep<ITEMS>, dim3(grid), dim3(kBlock), 0, s, p, a.values, a.keysA,
-                     a.ridsA, a.keysB, a.ridsB, a.n, a.rid_base, a.st, a.sort_status,
-                     (uint64_t)tiles * 256, tiles, &a.counts->fault);
+  hipLaunchKernelGGL((k_onesweep<kSortThreads, kSortItems>), dim3(grid), dim3(kSortThreads), 0,
+                     s, p, a.values, a.keysA, a.ridsA, a.keysB, a.ridsB, a.n, a.rid_base, a.st,
+                     a.sort_status, (uint64_t)tiles * 256, tiles, &a.counts->fault);
 }
 
 uint64_t sort_status_words(uint64_t n) {
-  uint64_t tiles = (n + kSortTile - 1) / kSortTile;
+  const uint64_t tiles = (n + kSortTile - 1) / kSortTile;
   return 2 * tiles * 256;
 }
-uint64_t tail_status_words(uint64_t n) {
-  return (n + kTailTile - 1) / kTailTile;
-}
+uint64_t tail_status_words(uint64_t n) { return (n + kTailTile - 1) / kTailTile; }
 uint64_t scan_status_words(uint64_t n) { return (n + kScanTile - 1) / kScanTile; }
 
 void launch_sort(const BuildArgs& a, hipStream_t s, LaunchHook* hook) {
-  uint32_t hb = (uint32_t)((a.n + 65535) / 65536);
+  uint32_t hb = (uint32_t)((a.n + 16383) / 16384);
   if (hb > kHistMaxBlocks) hb = kHistMaxBlocks;
   if (hb == 0) hb = 1;
-  int vec4 = ((uintptr_t)a.values % 16) == 0;
-  uint64_t tiles = (a.n + kSortTile - 1) / kSortTile;
-  hipLaunchKernelGGL(k_histogram, dim3(hb), dim3(kBlock), 0, s, a.values, a.n, vec4,
-                     a.part_hist, a.part_max, a.st, a.sort_status, tiles * 256,
-                     a.tail_status, tail_status_words(a.n), a.counts);
-  hipLaunchKernelGGL(k_hist_scan, dim3(1), dim3(kBlock), 0, s, a.part_hist, a.part_max, hb,
-                     a.n, a.st, a.counts);
+  const int vec4 = ((uintptr_t)a.values % 16) == 0;
+  const uint64_t tiles = (a.n + kSortTile - 1) / kSortTile;
+  hipLaunchKernelGGL(k_fill32, dim3(1), dim3(kBlock), 0, s, reinterpret_cast<uint32_t*>(a.st),
+                     kSortHeaderWords, 0u);
+  hipLaunchKernelGGL(k_histogram, dim3(hb), dim3(kBlock), 0, s, a.values, a.n, vec4, a.st,
+                     a.sort_status, tiles * 256, a.tail_status, tail_status_words(a.n),
+                     a.counts);
   if (hook) hook->mark(s);
   for (int p = 0; p < 4; ++p) {
-    launch_onesweep<kSortItems>(a, p, s);
+    launch_onesweep(a, p, s);
     if (hook) hook->mark(s);
   }
 }
 
 void launch_tail(const BuildArgs& a, hipStream_t s, LaunchHook* hook) {
-  uint32_t tiles = (uint32_t)((a.n + kTailTile - 1) / kTailTile);
-  uint32_t grid = tiles < a.max_resident ? tiles : a.max_resident;
+  const uint32_t tiles = (uint32_t)((a.n + kTailTile - 1) / kTailTile);
+  const uint32_t cap = WAH_STATIC_TILES ? tiles : a.max_resident * 256 / kTailThreads;
+  const uint32_t grid = tiles < cap ? tiles : cap;
   if (grid) {
-    hipLaunchKernelGGL(k_tail<kTailItems>, dim3(grid), dim3(kBlock), 0, s, a.st, a.values,
-                       a.keysA, a.ridsA, a.keysB, a.ridsB, a.n, a.rid_base, a.out_keys,
-                       a.out_offsets, a.out_words, a.tail_status, tiles,
+    hipLaunchKernelGGL((k_tail<kTailThreads, kTailItems>), dim3(grid), dim3(kTailThreads), 0, s,
+                       a.st, a.values, a.keysA, a.ridsA, a.keysB, a.ridsB, a.n, a.rid_base,
+                       a.out_keys, a.out_offsets, a.out_words, a.tail_status, tiles,
                        &a.st->tile_counter[kCounterTail], a.counts);
   }
   if (hook) hook->mark(s);

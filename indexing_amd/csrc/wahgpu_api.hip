@@ -93,8 +93,6 @@ uint64_t build_scratch_bytes(uint64_t n) {
   uint64_t b = 0;
   b += 4 * align_up(n * 4);                                  // keysA ridsA keysB ridsB
   b += align_up(sizeof(wah::SortState));
-  b += align_up(wah::kHistMaxBlocks * 1024 * 4);
-  b += align_up(wah::kHistMaxBlocks * 4);
   b += align_up(wah::sort_status_words(n) * 4);
   b += align_up(wah::tail_status_words(n) * 8);
   return b;
@@ -106,8 +104,6 @@ void carve_build(wahgpu_ctx* c, BuildArgs* a, uint64_t n) {
   a->keysB = take<uint32_t>(c, n);
   a->ridsB = take<uint32_t>(c, n);
   a->st = take<wah::SortState>(c, 1);
-  a->part_hist = take<uint32_t>(c, wah::kHistMaxBlocks * 1024);
-  a->part_max = take<uint32_t>(c, wah::kHistMaxBlocks);
   a->sort_status = take<uint32_t>(c, wah::sort_status_words(n));
   a->tail_status = take<uint64_t>(c, wah::tail_status_words(n));
   a->max_resident = c->max_resident;

@@ -55,26 +55,40 @@ __device__ __forceinline__ uint64_t shfl64(uint64_t x, int src) {
   return ((uint64_t)hi << 32) | lo;
 }
 
-// Tile scheduling for the single-pass kernels.  By default a block takes the
-// next tile from a global counter, so a tile is only started after every
-// lower tile was taken by a running block (look-back always progresses).
-// WAH_STATIC_TILES=1 maps tile = blockIdx.x instead (one tile per block),
-// which relies on workgroups being dispatched in order.
+// Tile scheduling for the single-pass kernels.  Tiles are handed out by
+// `lanes` counters (one per 256-byte line, kLaneStride words apart): block b
+// draws from counter b % lanes and gets tiles lane, lane + lanes, ... in
+// increasing order.  A global atomic on one address costs ~11 ns on MI355X
+// and they serialize, so spreading the draws over lanes removes that floor.
+// Every tile below a drawn tile was drawn earlier by a running block of its
+// lane, so look-back waits always end.  WAH_STATIC_TILES=1 maps
+// tile = blockIdx.x instead (one tile per block), relying on in-order dispatch.
 #ifndef WAH_STATIC_TILES
 #define WAH_STATIC_TILES 0
 #endif
-__device__ __forceinline__ uint32_t next_tile(uint32_t* counter, uint32_t* s_tile, uint32_t& iter) {
+constexpr uint32_t kTileLanes = 8;
+constexpr uint32_t kLaneStride = 64;
+__device__ __forceinline__ uint32_t next_tile(uint32_t* counters, uint32_t lanes,
+                                              uint32_t* s_tile, uint32_t& iter) {
 #if WAH_STATIC_TILES
-  (void)counter;
+  (void)counters;
+  (void)lanes;
   (void)s_tile;
   __syncthreads();
   return iter++ == 0 ? blockIdx.x : 0xFFFFFFFFu;
 #else
   (void)iter;
-  if (threadIdx.x == 0) *s_tile = atomicAdd(counter, 1u);
+  if (threadIdx.x == 0) {
+    const uint32_t lane = blockIdx.x % lanes;
+    *s_tile = atomicAdd(&counters[lane * kLaneStride], 1u) * lanes + lane;
+  }
   __syncthreads();
   return *s_tile;
 #endif
+}
+// Counter lanes for a grid of `grid` blocks.
+__host__ __device__ __forceinline__ uint32_t tile_lanes(uint32_t grid) {
+  return grid < kTileLanes ? (grid ? grid : 1u) : kTileLanes;
 }
 
 // Inclusive uint32 prefix sum across the wave with DPP row shifts and row
@@ -193,6 +207,47 @@ __device__ __forceinline__ uint32_t wait_status(uint32_t* p, uint32_t* fault) {
 }
 __device__ __forceinline__ uint64_t wait_status(uint64_t* p, uint32_t* fault) {
   return wait_status_t<uint64_t, 62>(p, fault);
+}
+
+// Exclusive prefix of one digit over the preceding tiles, reading up to
+// WAH_LOOKAHEAD predecessors per round trip (32-bit words: flag << 30 | count).
+#ifndef WAH_LOOKAHEAD
+#define WAH_LOOKAHEAD 4
+#endif
+__device__ __forceinline__ uint32_t lookback_digit(uint32_t* status, uint32_t tile,
+                                                   uint32_t stride, uint32_t* fault) {
+  constexpr int LA = WAH_LOOKAHEAD;
+  uint32_t acc = 0;
+  int64_t at = (int64_t)tile - 1;  // newest tile not yet summed
+  uint32_t spins = 0;
+  while (at >= 0) {
+    uint32_t s[LA];
+#pragma unroll
+    for (int k = 0; k < LA; ++k)
+      s[k] = at - k >= 0 ? ld_agent(&status[(uint64_t)(at - k) * stride]) : (kFlagPrefix << 30);
+    int k = 0;
+    bool done = false;
+#pragma unroll
+    for (int j = 0; j < LA; ++j) {
+      if (done || k != j) continue;
+      const uint32_t f = s[j] >> 30;
+      if (f == 0) continue;              // not published yet: poll again from here
+      acc += s[j] & 0x3FFFFFFFu;
+      ++k;
+      if (f == kFlagPrefix) done = true;
+    }
+    if (done) break;
+    at -= k;
+    if (k == 0) {
+      if (++spins >= kSpinLimit) {
+        atomicOr(fault, 1u);
+        break;
+      }
+      if ((spins & 1023u) == 0 && ld_agent(fault)) break;
+      __builtin_amdgcn_s_sleep(1);
+    }
+  }
+  return acc;
 }
 
 // Decoupled look-back over 64-bit status words (flag << 62 | payload).

@@ -49,19 +49,24 @@ struct PassInfo {
   uint32_t ordinal;     // index among active passes
 };
 
+constexpr uint32_t kHistCopies = 32;        // striped copies of the global histogram
+constexpr uint32_t kHistCopyStride = 1088;  // words between copies (4 KiB + 256 B)
+constexpr uint32_t kLaneWords = 8 * 64;     // tile-counter lanes of one kernel
+
 struct SortState {
   // --- zeroed before every build (kSortHeaderWords words) ---
-  uint32_t hist[4][256];          // digit histograms of the column
-  uint32_t done;                  // histogram blocks finished
+  uint32_t hist[kHistCopies][kHistCopyStride];  // [copy][digit * 256 + bin]
+  uint32_t tiles[kCounters][kLaneWords];        // tile-counter lanes per kernel
+  uint32_t done;                                // histogram blocks finished
   uint32_t maxval;
-  uint32_t tile_counter[kCounters];
   // --- written by the last histogram block ---
   uint32_t digit_offset[4][256];  // exclusive prefix of the digit histogram
   PassInfo pass[4];
   uint32_t num_active;
   uint32_t final_buf;             // buffer holding the sorted pairs
 };
-constexpr uint64_t kSortHeaderWords = 4 * 256 + 2 + kCounters;
+constexpr uint64_t kSortHeaderWords =
+    (uint64_t)kHistCopies * kHistCopyStride + (uint64_t)kCounters * kLaneWords + 2;
 
 struct BuildArgs {
   const uint32_t* values;
@@ -79,7 +84,7 @@ struct BuildArgs {
 
 struct ScanArgs {
   uint64_t* status;        // scan_status_words(n), zeroed
-  uint32_t* tile_counter;  // zeroed
+  uint32_t* tile_counter;  // kLaneWords tile-counter lanes, zeroed
   uint32_t* fault;         // zeroed; nonzero after a failed wait / bad index
   uint32_t max_resident;
 };

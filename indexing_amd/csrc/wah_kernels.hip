@@ -109,9 +109,11 @@ __global__ __launch_bounds__(kBlock) void k_histogram(
   }
   if ((t % kWave) == 0) s_max[t / kWave] = vmax;
   __syncthreads();
-  // fold into the column histogram (st->hist was zeroed by the previous launch)
+  // fold into one striped copy of the column histogram (zeroed by the
+  // previous launch); copies sit on different memory channels
+  uint32_t* copy = st->hist[blockIdx.x % kHistCopies];
   for (int i = t; i < 1024; i += kBlock)
-    if (h[i]) atomicAdd(&st->hist[0][0] + i, h[i]);
+    if (h[i]) atomicAdd(&copy[i], h[i]);
   if (t == 0) {
     uint32_t m = 0;
     for (int i = 0; i < kWaves; ++i) m = s_max[i] > m ? s_max[i] : m;
@@ -126,7 +128,8 @@ __global__ __launch_bounds__(kBlock) void k_histogram(
   // The last block: digit offsets and the pass plan.
   __threadfence();
   for (int p = 0; p < 4; ++p) {
-    const uint32_t c = ld_agent(&st->hist[p][t]);
+    uint32_t c = 0;
+    for (uint32_t k = 0; k < kHistCopies; ++k) c += ld_agent(&st->hist[k][p * 256 + t]);
     uint64_t total;
     const uint64_t ex = block_exclusive<OpAdd>(c, s_scan, &total);
     st->digit_offset[p][t] = (uint32_t)ex;
@@ -161,11 +164,11 @@ __global__ __launch_bounds__(kBlock) void k_histogram(
 // one LDS buffer in digit order and written out in runs per digit.
 // ---------------------------------------------------------------------------
 template <int THREADS, int ITEMS>
-__global__ __launch_bounds__(THREADS, 2) void k_onesweep(
+__global__ __launch_bounds__(THREADS, 4) void k_onesweep(
     int p, const uint32_t* __restrict__ values, uint32_t* keysA, uint32_t* ridsA,
     uint32_t* keysB, uint32_t* ridsB, uint64_t n, uint32_t rid_base,
     SortState* __restrict__ st, uint32_t* status_base, uint64_t status_stride,
-    uint32_t num_tiles, uint32_t* fault) {
+    uint32_t num_tiles, uint32_t lanes, uint32_t* fault) {
   constexpr int TILE = THREADS * ITEMS;
   constexpr int WAVES = THREADS / kWave;
   static_assert(THREADS >= 256 && THREADS % 256 == 0, "one thread per digit");
@@ -193,7 +196,7 @@ __global__ __launch_bounds__(THREADS, 2) void k_onesweep(
   uint32_t iter = 0;
   for (;;) {
     for (int i = t; i < WAVES * 256; i += THREADS) (&s_wcnt[0][0])[i] = 0;
-    const uint32_t tile = next_tile(&st->tile_counter[p], &s_tile, iter);
+    const uint32_t tile = next_tile(st->tiles[p], lanes, &s_tile, iter);
     if (tile >= num_tiles) break;
     const uint64_t base = (uint64_t)tile * TILE;
     if (t < 256) status_next[(uint64_t)tile * 256 + t] = 0;  // next active pass's state
@@ -246,13 +249,7 @@ __global__ __launch_bounds__(THREADS, 2) void k_onesweep(
       s_excl[t] = inc - total;  // within this wave's 64 digits
       uint32_t prev = 0;
       if (tile > 0) {
-        int64_t tt = (int64_t)tile - 1;
-        while (tt >= 0) {
-          const uint32_t s = wait_status(&status[(uint64_t)tt * 256 + t], fault);
-          prev += s & 0x3FFFFFFFu;
-          if ((s >> 30) != kFlagAgg) break;  // prefix found (or gave up)
-          --tt;
-        }
+        prev = lookback_digit(status + t, tile, 256, fault);
         st_agent(my, (kFlagPrefix << 30) | (prev + total));
       }
       s_glob[t] = goff + prev;  // the tile-local offset is subtracted below
@@ -342,7 +339,7 @@ __global__ __launch_bounds__(THREADS) void k_tail(
     const uint32_t* ridsA, const uint32_t* keysB, const uint32_t* ridsB, uint64_t n,
     uint32_t rid_base, uint32_t* __restrict__ out_keys, uint32_t* __restrict__ out_offsets,
     uint32_t* __restrict__ out_words, uint64_t* status, uint32_t num_tiles,
-    uint32_t* tile_counter, wahgpu_counts_dev* __restrict__ counts) {
+    uint32_t* tile_counter, uint32_t lanes, wahgpu_counts_dev* __restrict__ counts) {
   constexpr int TILE = THREADS * ITEMS;
   constexpr int WAVES = THREADS / kWave;
   constexpr int SUB = kWave * ITEMS;     // pairs per wave
@@ -363,7 +360,7 @@ __global__ __launch_bounds__(THREADS) void k_tail(
 
   uint32_t iter = 0;
   for (;;) {
-    const uint32_t tile = next_tile(tile_counter, &s_tile, iter);
+    const uint32_t tile = next_tile(tile_counter, lanes, &s_tile, iter);
     if (tile >= num_tiles) break;
     const uint64_t base = (uint64_t)tile * TILE;
     // stage pairs [base - 1, base + TILE + HALO - 1) with 16-byte loads
@@ -555,16 +552,16 @@ __global__ __launch_bounds__(kBlock) void k_merge(
     const uint32_t* __restrict__ vals, const uint32_t* __restrict__ chids,
     const uint32_t* __restrict__ lits, uint64_t n, uint32_t* __restrict__ out_v,
     uint32_t* __restrict__ out_c, uint32_t* __restrict__ out_l, uint64_t* status,
-    uint32_t num_tiles, uint32_t* tile_counter, uint64_t* result, uint32_t* fault) {
+    uint32_t num_tiles, uint32_t* tile_counter, uint32_t lanes, uint64_t* result,
+    uint32_t* fault) {
   constexpr int TILE = kBlock * ITEMS;
   __shared__ uint64_t s_scan[kWaves];
   __shared__ uint64_t s_prefix;
   __shared__ uint32_t s_tile;
   const int t = threadIdx.x;
+  uint32_t iter = 0;
   for (;;) {
-    if (t == 0) s_tile = atomicAdd(tile_counter, 1u);
-    __syncthreads();
-    const uint32_t tile = s_tile;
+    const uint32_t tile = next_tile(tile_counter, lanes, &s_tile, iter);
     if (tile >= num_tiles) break;
     const uint64_t g0 = (uint64_t)tile * TILE + (uint64_t)t * ITEMS;
     uint32_t headmask = 0;
@@ -615,17 +612,16 @@ template <int ITEMS, int MODE>
 __global__ __launch_bounds__(kBlock) void k_compact(
     const uint32_t* __restrict__ a, const uint32_t* __restrict__ b, uint64_t n,
     uint32_t* __restrict__ out, uint64_t* status, uint32_t num_tiles,
-    uint32_t* tile_counter, uint64_t* result, uint32_t* fault) {
+    uint32_t* tile_counter, uint32_t lanes, uint64_t* result, uint32_t* fault) {
   constexpr int TILE = kBlock * ITEMS;
   const uint64_t cap = MODE == 0 ? n : 2 * n;
   __shared__ uint64_t s_scan[kWaves];
   __shared__ uint64_t s_prefix;
   __shared__ uint32_t s_tile;
   const int t = threadIdx.x;
+  uint32_t iter = 0;
   for (;;) {
-    if (t == 0) s_tile = atomicAdd(tile_counter, 1u);
-    __syncthreads();
-    const uint32_t tile = s_tile;
+    const uint32_t tile = next_tile(tile_counter, lanes, &s_tile, iter);
     if (tile >= num_tiles) break;
     const uint64_t g0 = (uint64_t)tile * TILE + (uint64_t)t * ITEMS;
     uint64_t cnt = 0;
@@ -672,17 +668,16 @@ template <int ITEMS>
 __global__ __launch_bounds__(kBlock) void k_colen(
     const uint32_t* __restrict__ vals, const uint32_t* __restrict__ fills, uint64_t k,
     uint32_t* __restrict__ out_keys, uint32_t* __restrict__ out_offsets, uint64_t* status,
-    uint32_t num_tiles, uint32_t* tile_counter, wahgpu_counts_dev* __restrict__ counts,
-    uint32_t* fault) {
+    uint32_t num_tiles, uint32_t* tile_counter, uint32_t lanes,
+    wahgpu_counts_dev* __restrict__ counts, uint32_t* fault) {
   constexpr int TILE = kBlock * ITEMS;
   __shared__ uint64_t s_scan[kWaves];
   __shared__ uint64_t s_prefix;
   __shared__ uint32_t s_tile;
   const int t = threadIdx.x;
+  uint32_t iter = 0;
   for (;;) {
-    if (t == 0) s_tile = atomicAdd(tile_counter, 1u);
-    __syncthreads();
-    const uint32_t tile = s_tile;
+    const uint32_t tile = next_tile(tile_counter, lanes, &s_tile, iter);
     if (tile >= num_tiles) break;
     const uint64_t g0 = (uint64_t)tile * TILE + (uint64_t)t * ITEMS;
     uint32_t W = 0, KH = 0;
@@ -732,17 +727,16 @@ template <int ITEMS, bool SEGMENTED>
 __global__ __launch_bounds__(kBlock) void k_scan(
     const uint32_t* __restrict__ in, const uint32_t* __restrict__ heads, uint64_t n,
     uint32_t* __restrict__ out, uint64_t* status, uint32_t num_tiles,
-    uint32_t* tile_counter, uint32_t* fault) {
+    uint32_t* tile_counter, uint32_t lanes, uint32_t* fault) {
   constexpr int TILE = kBlock * ITEMS;
   __shared__ uint64_t s_scan[kWaves];
   __shared__ uint64_t s_prefix;
   __shared__ uint32_t s_tile;
   using Op = typename std::conditional<SEGMENTED, OpSegAdd, OpAdd32>::type;
   const int t = threadIdx.x;
+  uint32_t iter = 0;
   for (;;) {
-    if (t == 0) s_tile = atomicAdd(tile_counter, 1u);
-    __syncthreads();
-    const uint32_t tile = s_tile;
+    const uint32_t tile = next_tile(tile_counter, lanes, &s_tile, iter);
     if (tile >= num_tiles) break;
     const uint64_t g0 = (uint64_t)tile * TILE + (uint64_t)t * ITEMS;
     uint64_t agg = Op::identity();
@@ -789,7 +783,8 @@ static void launch_onesweep(const BuildArgs& a, int p, hipStream_t s) {
   if (grid == 0) grid = 1;
   hipLaunchKernelGGL((k_onesweep<kSortThreads, kSortItems>), dim3(grid), dim3(kSortThreads), 0,
                      s, p, a.values, a.keysA, a.ridsA, a.keysB, a.ridsB, a.n, a.rid_base, a.st,
-                     a.sort_status, (uint64_t)tiles * 256, tiles, &a.counts->fault);
+                     a.sort_status, (uint64_t)tiles * 256, tiles, tile_lanes(grid),
+                     &a.counts->fault);
 }
 
 uint64_t sort_status_words(uint64_t n) {
@@ -805,7 +800,7 @@ void launch_sort(const BuildArgs& a, hipStream_t s, LaunchHook* hook) {
   if (hb == 0) hb = 1;
   const int vec4 = ((uintptr_t)a.values % 16) == 0;
   const uint64_t tiles = (a.n + kSortTile - 1) / kSortTile;
-  hipLaunchKernelGGL(k_fill32, dim3(1), dim3(kBlock), 0, s, reinterpret_cast<uint32_t*>(a.st),
+  hipLaunchKernelGGL(k_fill32, dim3(64), dim3(kBlock), 0, s, reinterpret_cast<uint32_t*>(a.st),
                      kSortHeaderWords, 0u);
   hipLaunchKernelGGL(k_histogram, dim3(hb), dim3(kBlock), 0, s, a.values, a.n, vec4, a.st,
                      a.sort_status, tiles * 256, a.tail_status, tail_status_words(a.n),
@@ -825,7 +820,7 @@ void launch_tail(const BuildArgs& a, hipStream_t s, LaunchHook* hook) {
     hipLaunchKernelGGL((k_tail<kTailThreads, kTailItems>), dim3(grid), dim3(kTailThreads), 0, s,
                        a.st, a.values, a.keysA, a.ridsA, a.keysB, a.ridsB, a.n, a.rid_base,
                        a.out_keys, a.out_offsets, a.out_words, a.tail_status, tiles,
-                       &a.st->tile_counter[kCounterTail], a.counts);
+                       a.st->tiles[kCounterTail], tile_lanes(grid), a.counts);
   }
   if (hook) hook->mark(s);
   uint32_t lg = (uint32_t)((a.n + kBlock - 1) / kBlock);
@@ -879,7 +874,8 @@ void launch_merge(const ScanArgs& a, const uint32_t* vals, const uint32_t* chids
   uint32_t tiles, grid = scan_grid(n, a.max_resident, &tiles);
   if (!grid) return;
   hipLaunchKernelGGL(k_merge<kScanItems>, dim3(grid), dim3(kBlock), 0, s, vals, chids, lits, n,
-                     ov, oc, ol, a.status, tiles, a.tile_counter, result, a.fault);
+                     ov, oc, ol, a.status, tiles, a.tile_counter, tile_lanes(grid), result,
+                     a.fault);
 }
 
 void launch_compact(const ScanArgs& a, const uint32_t* data, const uint32_t* valid, uint64_t n,
@@ -888,10 +884,12 @@ void launch_compact(const ScanArgs& a, const uint32_t* data, const uint32_t* val
   if (!grid) return;
   if (mode == 0)
     hipLaunchKernelGGL((k_compact<kScanItems, 0>), dim3(grid), dim3(kBlock), 0, s, data, valid,
-                       n, out, a.status, tiles, a.tile_counter, result, a.fault);
+                       n, out, a.status, tiles, a.tile_counter, tile_lanes(grid), result,
+                       a.fault);
   else
     hipLaunchKernelGGL((k_compact<kScanItems, 1>), dim3(grid), dim3(kBlock), 0, s, data, valid,
-                       n, out, a.status, tiles, a.tile_counter, result, a.fault);
+                       n, out, a.status, tiles, a.tile_counter, tile_lanes(grid), result,
+                       a.fault);
 }
 
 void launch_colen(const ScanArgs& a, const uint32_t* vals, const uint32_t* fills, uint64_t k,
@@ -900,7 +898,8 @@ void launch_colen(const ScanArgs& a, const uint32_t* vals, const uint32_t* fills
   uint32_t tiles, grid = scan_grid(k, a.max_resident, &tiles);
   if (grid)
     hipLaunchKernelGGL(k_colen<kScanItems>, dim3(grid), dim3(kBlock), 0, s, vals, fills, k, keys,
-                       offsets, a.status, tiles, a.tile_counter, counts, a.fault);
+                       offsets, a.status, tiles, a.tile_counter, tile_lanes(grid), counts,
+                       a.fault);
   uint32_t lg = grid_for(k);
   if (lg > 1024) lg = 1024;
   hipLaunchKernelGGL(k_lengths, dim3(lg), dim3(kBlock), 0, s, offsets, lengths, counts, k);
@@ -912,10 +911,10 @@ void launch_scan(const ScanArgs& a, const uint32_t* in, const uint32_t* heads, u
   if (!grid) return;
   if (segmented)
     hipLaunchKernelGGL((k_scan<kScanItems, true>), dim3(grid), dim3(kBlock), 0, s, in, heads, n,
-                       out, a.status, tiles, a.tile_counter, a.fault);
+                       out, a.status, tiles, a.tile_counter, tile_lanes(grid), a.fault);
   else
     hipLaunchKernelGGL((k_scan<kScanItems, false>), dim3(grid), dim3(kBlock), 0, s, in, heads, n,
-                       out, a.status, tiles, a.tile_counter, a.fault);
+                       out, a.status, tiles, a.tile_counter, tile_lanes(grid), a.fault);
 }
 
 }  // namespace wah

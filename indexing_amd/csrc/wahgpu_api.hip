@@ -109,23 +109,29 @@ void carve_build(wahgpu_ctx* c, BuildArgs* a, uint64_t n) {
   a->max_resident = c->max_resident;
 }
 
-// Scan scratch: [counter u32, pad, result u64, status u64 x tiles] zeroed.
+// Scan scratch, zeroed before use: tile-counter lanes (kLaneWords words),
+// then [fault u32, pad u32, result u64], then the u64 status words.
 struct ScanScratch {
   uint32_t* base;
   uint64_t words;
   ScanArgs args;
+  uint32_t* header;  // fault, pad, result lo, result hi
   uint64_t* result;
 };
-uint64_t scan_scratch_bytes(uint64_t n) { return align_up(16 + wah::scan_status_words(n) * 8); }
+constexpr uint64_t kScanHeadWords = wah::kLaneWords + 8;
+uint64_t scan_scratch_bytes(uint64_t n) {
+  return align_up(kScanHeadWords * 4 + wah::scan_status_words(n) * 8);
+}
 ScanScratch carve_scan(wahgpu_ctx* c, uint64_t n) {
   ScanScratch s;
-  uint64_t bytes = 16 + wah::scan_status_words(n) * 8;
+  uint64_t bytes = kScanHeadWords * 4 + wah::scan_status_words(n) * 8;
   s.base = take<uint32_t>(c, bytes / 4);
   s.words = bytes / 4;
   s.args.tile_counter = s.base;
-  s.args.fault = s.base + 1;
-  s.result = reinterpret_cast<uint64_t*>(s.base + 2);
-  s.args.status = reinterpret_cast<uint64_t*>(s.base + 4);
+  s.header = s.base + wah::kLaneWords;
+  s.args.fault = s.header;
+  s.result = reinterpret_cast<uint64_t*>(s.header + 2);
+  s.args.status = reinterpret_cast<uint64_t*>(s.base + kScanHeadWords);
   s.args.max_resident = c->max_resident;
   return s;
 }
@@ -150,14 +156,14 @@ int read_counts(wahgpu_ctx* c, const wahgpu_counts* d_counts, wahgpu_counts* out
   return WAHGPU_OK;
 }
 
-// Reads a scan scratch header [counter, fault, result] and its result.
-int read_scan_result(wahgpu_ctx* c, const uint32_t* base, uint64_t* out) {
-  HIPCHK(c, hipMemcpyAsync(c->pinned_counts, base, 16, hipMemcpyDeviceToHost, c->stream));
+// Reads a scan scratch header [fault, pad, result] and its result.
+int read_scan_result(wahgpu_ctx* c, const uint32_t* header, uint64_t* out) {
+  HIPCHK(c, hipMemcpyAsync(c->pinned_counts, header, 16, hipMemcpyDeviceToHost, c->stream));
   HIPCHK(c, hipStreamSynchronize(c->stream));
   uint32_t hdr[4];
   memcpy(hdr, c->pinned_counts, 16);
-  if (hdr[1])
-    return fail(c, WAHGPU_EHIP, "kernel reported fault " + std::to_string(hdr[1]) +
+  if (hdr[0])
+    return fail(c, WAHGPU_EHIP, "kernel reported fault " + std::to_string(hdr[0]) +
                                     " (look-back wait gave up or index out of range)");
   memcpy(out, &hdr[2], 8);
   return WAHGPU_OK;
@@ -504,7 +510,7 @@ int wahgpu_build_staged(wahgpu_ctx* c, const uint32_t* values, uint64_t n, uint3
   wah::launch_merge(sm.args, s_keys, chids, lits, n, mv, mc, ml, sm.result, c->stream);
   LAUNCHCHK(c);
   uint64_t k = 0;
-  rc = read_scan_result(c, sm.base, &k);  // k sizes the next launches (src/phases.cpp:890-903)
+  rc = read_scan_result(c, sm.header, &k);  // k sizes the next launches (src/phases.cpp:890-903)
   if (rc) return rc;
   m.mark(c->stream);  // 4 merge
   wah::launch_fills(mv, mc, k, fills, c->stream);
@@ -514,7 +520,7 @@ int wahgpu_build_staged(wahgpu_ctx* c, const uint32_t* values, uint64_t n, uint3
   wah::launch_compact(sf.args, fills, ml, k, 1, index, sf.result, c->stream);
   LAUNCHCHK(c);
   uint64_t L = 0;
-  rc = read_scan_result(c, sf.base, &L);
+  rc = read_scan_result(c, sf.header, &L);
   if (rc) return rc;
   m.mark(c->stream);  // 6 fuse
   wah::launch_fill32(sc.base, sc.words, 0, c->stream);
@@ -522,7 +528,7 @@ int wahgpu_build_staged(wahgpu_ctx* c, const uint32_t* values, uint64_t n, uint3
   LAUNCHCHK(c);
   wahgpu_counts cc;
   uint64_t unused;
-  rc = read_scan_result(c, sc.base, &unused);  // fault word of the column kernel
+  rc = read_scan_result(c, sc.header, &unused);  // fault word of the column kernel
   if (rc) return rc;
   rc = read_counts(c, d_counts, &cc);
   if (rc) return rc;
@@ -610,7 +616,7 @@ int wahgpu_merge_lit_by_val_chids(wahgpu_ctx* c, const uint32_t* d_values,
   wah::launch_merge(s.args, d_values, d_chids, d_lits, n, d_values_out, d_chids_out, d_lits_out,
                     s.result, c->stream);
   LAUNCHCHK(c);
-  return read_scan_result(c, s.base, k);
+  return read_scan_result(c, s.header, k);
 }
 
 int wahgpu_produce_fills(wahgpu_ctx* c, const uint32_t* d_values, const uint32_t* d_chids,
@@ -636,7 +642,7 @@ int wahgpu_fuse_fill_literals(wahgpu_ctx* c, const uint32_t* d_fills, const uint
   wah::launch_fill32(s.base, s.words, 0, c->stream);
   wah::launch_compact(s.args, d_fills, d_lits, k, 1, d_index, s.result, c->stream);
   LAUNCHCHK(c);
-  return read_scan_result(c, s.base, index_length);
+  return read_scan_result(c, s.header, index_length);
 }
 
 int wahgpu_compute_column_length(wahgpu_ctx* c, const uint32_t* d_values,
@@ -660,7 +666,7 @@ int wahgpu_compute_column_length(wahgpu_ctx* c, const uint32_t* d_values,
   LAUNCHCHK(c);
   wahgpu_counts cc;
   uint64_t unused;
-  rc = read_scan_result(c, s.base, &unused);
+  rc = read_scan_result(c, s.header, &unused);
   if (rc) return rc;
   rc = read_counts(c, d_counts, &cc);
   if (rc) return rc;
@@ -681,7 +687,7 @@ int wahgpu_stream_compaction(wahgpu_ctx* c, const uint32_t* d_data, const uint32
   wah::launch_fill32(s.base, s.words, 0, c->stream);
   wah::launch_compact(s.args, d_data, d_valid, n, 0, d_out, s.result, c->stream);
   LAUNCHCHK(c);
-  return read_scan_result(c, s.base, count);
+  return read_scan_result(c, s.header, count);
 }
 
 int wahgpu_exclusive_scan(wahgpu_ctx* c, const uint32_t* d_in, uint64_t n, uint32_t* d_out) {

@@ -91,6 +91,21 @@ __host__ __device__ __forceinline__ uint32_t tile_lanes(uint32_t grid) {
   return grid < kTileLanes ? (grid ? grid : 1u) : kTileLanes;
 }
 
+// Asynchronous copy of 16 bytes per lane from global memory straight into
+// LDS (gfx950 global_load_lds_dwordx4): lane l's bytes land at
+// lds_wave_base + 16 * l.  The base must be the same for the whole wave.
+// Completion is tracked by vmcnt; see wait_vm().
+__device__ __forceinline__ void load_lds16(const void* g, uint32_t* lds_wave_base) {
+  __builtin_amdgcn_global_load_lds(g, (__attribute__((address_space(3))) void*)lds_wave_base,
+                                   16, 0, 0);
+}
+// s_waitcnt vmcnt(N) with lgkmcnt/expcnt left alone (gfx9 encoding).
+template <int N>
+__device__ __forceinline__ void wait_vm() {
+  static_assert(N >= 0 && N < 64, "vmcnt range");
+  __builtin_amdgcn_s_waitcnt((N & 15) | ((N >> 4) << 14) | (7 << 4) | (15 << 8));
+}
+
 // Inclusive uint32 prefix sum across the wave with DPP row shifts and row
 // broadcasts (rows of 16 lanes, then rows 1/3 and 2/3 pick up the totals).
 __device__ __forceinline__ uint32_t wave_incl_add(uint32_t x) {

@@ -24,6 +24,10 @@ namespace wah {
 #ifndef WAH_TAIL_ITEMS
 #define WAH_TAIL_ITEMS 8
 #endif
+#ifndef WAH_SORT_WAVES_PER_EU
+#define WAH_SORT_WAVES_PER_EU 4
+#endif
+
 constexpr int kSortThreads = WAH_SORT_THREADS;  // threads of a radix-pass block
 constexpr int kSortItems = WAH_SORT_ITEMS;      // keys per thread in a radix pass
 constexpr int kSortTile = kSortThreads * kSortItems;
@@ -57,16 +61,15 @@ struct SortState {
   // --- zeroed before every build (kSortHeaderWords words) ---
   uint32_t hist[kHistCopies][kHistCopyStride];  // [copy][digit * 256 + bin]
   uint32_t tiles[kCounters][kLaneWords];        // tile-counter lanes per kernel
-  uint32_t done;                                // histogram blocks finished
   uint32_t maxval;
-  // --- written by the last histogram block ---
+  // --- written by k_plan ---
   uint32_t digit_offset[4][256];  // exclusive prefix of the digit histogram
   PassInfo pass[4];
   uint32_t num_active;
   uint32_t final_buf;             // buffer holding the sorted pairs
 };
 constexpr uint64_t kSortHeaderWords =
-    (uint64_t)kHistCopies * kHistCopyStride + (uint64_t)kCounters * kLaneWords + 2;
+    (uint64_t)kHistCopies * kHistCopyStride + (uint64_t)kCounters * kLaneWords + 1;
 
 struct BuildArgs {
   const uint32_t* values;

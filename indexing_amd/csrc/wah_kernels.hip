@@ -3,9 +3,9 @@
 // Fused build (what wahgpu_build runs), for a column of n uint32 values:
 //   k_fill32      zeroes the sort header (histogram, counters)
 //   k_histogram   one read of the values: histograms of all four 8-bit
-//                 digits and the maximum, zeroing of look-back state; the
-//                 last block scans the digit offsets and switches off passes
-//                 whose digit is the same for every value (no host sync)
+//                 digits and the maximum, zeroing of look-back state
+//   k_plan        digit offsets; passes whose digit is the same for every
+//                 value are switched off (on the device, no host sync)
 //   k_onesweep x4 stable LSD radix pass per active digit: wave64 ballot
 //                 ranking, LDS staging, decoupled look-back over tiles
 //   k_tail        sorted (value, rid) -> chunk ids, OR-merged literals, fills,
@@ -53,9 +53,6 @@ __global__ __launch_bounds__(kBlock) void k_histogram(
     wahgpu_counts_dev* __restrict__ counts) {
   __shared__ uint32_t h[4 * 256];
   __shared__ uint32_t s_max[kWaves];
-  __shared__ uint64_t s_scan[kWaves];
-  __shared__ int s_trivial[4];
-  __shared__ int s_last;
   const int t = threadIdx.x;
   for (int i = t; i < 1024; i += kBlock) h[i] = 0;
   if (blockIdx.x == 0 && t == 0) {
@@ -119,17 +116,21 @@ __global__ __launch_bounds__(kBlock) void k_histogram(
     for (int i = 0; i < kWaves; ++i) m = s_max[i] > m ? s_max[i] : m;
     atomicMax(&st->maxval, m);
   }
-  __threadfence();
-  __syncthreads();
-  if (t == 0) s_last = atomicAdd(&st->done, 1u) == gridDim.x - 1;
+}
+
+// One block, after the histogram: digit offsets and the pass plan.  A kernel
+// of its own so the histogram blocks need no device-scope fences (on gfx950
+// each one writes back and invalidates the L2).
+__global__ __launch_bounds__(kBlock) void k_plan(SortState* __restrict__ st, uint64_t n,
+                                                 wahgpu_counts_dev* __restrict__ counts) {
+  __shared__ uint64_t s_scan[kWaves];
+  __shared__ int s_trivial[4];
+  const int t = threadIdx.x;
   if (t < 4) s_trivial[t] = 0;
   __syncthreads();
-  if (!s_last) return;
-  // The last block: digit offsets and the pass plan.
-  __threadfence();
   for (int p = 0; p < 4; ++p) {
     uint32_t c = 0;
-    for (uint32_t k = 0; k < kHistCopies; ++k) c += ld_agent(&st->hist[k][p * 256 + t]);
+    for (uint32_t k = 0; k < kHistCopies; ++k) c += st->hist[k][p * 256 + t];
     uint64_t total;
     const uint64_t ex = block_exclusive<OpAdd>(c, s_scan, &total);
     st->digit_offset[p][t] = (uint32_t)ex;
@@ -151,7 +152,7 @@ __global__ __launch_bounds__(kBlock) void k_histogram(
     }
     st->num_active = a;
     st->final_buf = (a == 0) ? kBufValues : ((a - 1) % 2 == 0 ? kBufA : kBufB);
-    counts->bound = ld_agent(&st->maxval);
+    counts->bound = st->maxval;
     counts->passes = a;
   }
 }
@@ -164,7 +165,7 @@ __global__ __launch_bounds__(kBlock) void k_histogram(
 // one LDS buffer in digit order and written out in runs per digit.
 // ---------------------------------------------------------------------------
 template <int THREADS, int ITEMS>
-__global__ __launch_bounds__(THREADS, 4) void k_onesweep(
+__global__ __launch_bounds__(THREADS, WAH_SORT_WAVES_PER_EU) void k_onesweep(
     int p, const uint32_t* __restrict__ values, uint32_t* keysA, uint32_t* ridsA,
     uint32_t* keysB, uint32_t* ridsB, uint64_t n, uint32_t rid_base,
     SortState* __restrict__ st, uint32_t* status_base, uint64_t status_stride,
@@ -201,21 +202,27 @@ __global__ __launch_bounds__(THREADS, 4) void k_onesweep(
     const uint64_t base = (uint64_t)tile * TILE;
     if (t < 256) status_next[(uint64_t)tile * 256 + t] = 0;  // next active pass's state
 
-    uint32_t key[ITEMS], rid[ITEMS], rank[ITEMS];
+    // Keys only for now: the rids are loaded once the keys are staged, so
+    // they never occupy registers during ranking and look-back.
+    uint32_t key[ITEMS], rank[ITEMS];
     const uint64_t wbase = base + (uint64_t)w * kWave * ITEMS + l;
+    const bool full = base + TILE <= n;
+    const uint32_t nvalid = full ? (uint32_t)ITEMS * kWave : (uint32_t)(n > wbase ? n - wbase : 0);
+    {
+      const uint32_t* kp = src_keys + wbase;  // one base, immediate offsets
+      if (full) {
 #pragma unroll
-    for (int i = 0; i < ITEMS; ++i) {
-      const uint64_t idx = wbase + (uint64_t)i * kWave;
-      const bool valid = idx < n;
-      key[i] = valid ? src_keys[idx] : 0u;
-      rid[i] = implicit ? rid_base + (uint32_t)idx : (valid ? src_rids[idx] : 0u);
+        for (int i = 0; i < ITEMS; ++i) key[i] = kp[i * kWave];
+      } else {
+#pragma unroll
+        for (int i = 0; i < ITEMS; ++i) key[i] = (uint32_t)(i * kWave) < nvalid ? kp[i * kWave] : 0u;
+      }
     }
     // Rank: peers = lanes with the same digit (8 ballots); rank = peers
     // below this lane + keys of that digit the wave ranked before.
 #pragma unroll
     for (int i = 0; i < ITEMS; ++i) {
-      const uint64_t idx = wbase + (uint64_t)i * kWave;
-      if (idx < n) {
+      if ((uint32_t)(i * kWave) < nvalid) {
         const uint32_t d = (key[i] >> shift) & 255u;
         uint64_t peers = ballot(true);
 #pragma unroll
@@ -266,14 +273,23 @@ __global__ __launch_bounds__(THREADS, 4) void k_onesweep(
     __syncthreads();
 #pragma unroll
     for (int i = 0; i < ITEMS; ++i) {
-      const uint64_t idx = wbase + (uint64_t)i * kWave;
-      if (idx < n) {
+      if ((uint32_t)(i * kWave) < nvalid) {
         const uint32_t d = (key[i] >> shift) & 255u;
         const uint32_t pos = s_excl[d] + s_wcnt[w][d] + rank[i];
         s_stage[pos] = key[i];
         s_digit[pos] = (uint8_t)d;
         rank[i] = pos;  // keep the LDS slot for the rids
       }
+    }
+    // rids: issued now, in flight while the keys go out
+    uint32_t rid[ITEMS];
+    if (implicit) {
+#pragma unroll
+      for (int i = 0; i < ITEMS; ++i) rid[i] = rid_base + (uint32_t)wbase + (uint32_t)(i * kWave);
+    } else {
+      const uint32_t* rp = src_rids + wbase;
+#pragma unroll
+      for (int i = 0; i < ITEMS; ++i) rid[i] = (uint32_t)(i * kWave) < nvalid ? rp[i * kWave] : 0u;
     }
     __syncthreads();
     const uint32_t tile_n = (n - base) < (uint64_t)TILE ? (uint32_t)(n - base) : (uint32_t)TILE;
@@ -288,10 +304,8 @@ __global__ __launch_bounds__(THREADS, 4) void k_onesweep(
     }
     __syncthreads();
 #pragma unroll
-    for (int i = 0; i < ITEMS; ++i) {
-      const uint64_t idx = wbase + (uint64_t)i * kWave;
-      if (idx < n) s_stage[rank[i]] = rid[i];
-    }
+    for (int i = 0; i < ITEMS; ++i)
+      if ((uint32_t)(i * kWave) < nvalid) s_stage[rank[i]] = rid[i];
     __syncthreads();
 #pragma unroll
     for (int j = 0; j < ITEMS; ++j) {
@@ -805,6 +819,7 @@ void launch_sort(const BuildArgs& a, hipStream_t s, LaunchHook* hook) {
   hipLaunchKernelGGL(k_histogram, dim3(hb), dim3(kBlock), 0, s, a.values, a.n, vec4, a.st,
                      a.sort_status, tiles * 256, a.tail_status, tail_status_words(a.n),
                      a.counts);
+  hipLaunchKernelGGL(k_plan, dim3(1), dim3(kBlock), 0, s, a.st, a.n, a.counts);
   if (hook) hook->mark(s);
   for (int p = 0; p < 4; ++p) {
     launch_onesweep(a, p, s);

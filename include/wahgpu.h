@@ -82,7 +82,8 @@ void wahgpu_destroy(wahgpu_ctx* ctx);
 const char* wahgpu_last_error(const wahgpu_ctx* ctx);
 void* wahgpu_stream(wahgpu_ctx* ctx);               /* the hipStream_t */
 /* Orders the context's work on a caller's hipStream_t (e.g. the framework's
- * current stream); NULL returns to the context's own stream. */
+ * current stream); NULL returns to the context's own stream, a blocking
+ * stream that is ordered with the legacy default stream (torch's default). */
 int wahgpu_set_stream(wahgpu_ctx* ctx, void* stream);
 int wahgpu_synchronize(wahgpu_ctx* ctx);
 uint64_t wahgpu_arena_bytes(const wahgpu_ctx* ctx); /* currently reserved */
@@ -125,6 +126,58 @@ int wahgpu_build_staged(wahgpu_ctx* ctx, const uint32_t* values, uint64_t n,
                         uint32_t* keys, uint32_t* offsets, uint32_t* lengths,
                         uint64_t keys_cap, uint32_t* words, uint64_t words_cap,
                         wahgpu_counts* counts, wahgpu_phase_times* times);
+
+/* ---- batched build: many columns in one launch set ----------------------
+ * The CAF indexer sends one column (slice) per worker request
+ * (src/gpu.cpp:194-227); this indexes any number of columns of uneven length
+ * and cardinality together: one histogram, one plan, four radix, one tail
+ * and one finish launch in all.  Radix launch a runs every column's a-th
+ * non-constant digit, with look-back chains per column; tiles never straddle
+ * columns.  Each column's result equals wahgpu_build_device of that column
+ * alone with the same rid_base. */
+typedef struct wahgpu_column {
+  uint64_t offset;    /* first value of the column in d_values */
+  uint64_t n;         /* values, at most WAHGPU_MAX_VALUES */
+  uint32_t rid_base;  /* row id of its first value */
+  uint32_t reserved;  /* 0 */
+} wahgpu_column;
+
+/* Device pointers.  Columns ascend by offset and do not overlap.  Column c
+ * writes keys, offsets and lengths to [offset, offset + n) of d_keys,
+ * d_offsets, d_lengths, its words to [2 offset, 2 offset + 2 n) of d_words and
+ * its counts to d_counts[c]; offsets are relative to the column's first word.
+ * values_len is the length of d_values (d_keys, d_offsets, d_lengths hold
+ * values_len entries, d_words 2 values_len).  Asynchronous on the context's
+ * stream; `cols` is consumed before the call returns. */
+int wahgpu_build_batch_device(wahgpu_ctx* ctx, const uint32_t* d_values,
+                              uint64_t values_len, const wahgpu_column* cols,
+                              uint32_t ncols, uint32_t* d_keys, uint32_t* d_offsets,
+                              uint32_t* d_lengths, uint32_t* d_words,
+                              wahgpu_counts* d_counts);
+
+/* Host pointers: column c is values[c][0 .. n[c]) with row ids from
+ * rid_base[c] (NULL: all 0); its outputs land in results[c] (capacities as
+ * for wahgpu_build).  Synchronous. */
+typedef struct wahgpu_result {
+  uint32_t* keys;
+  uint32_t* offsets;
+  uint32_t* lengths;
+  uint64_t keys_cap;
+  uint32_t* words;
+  uint64_t words_cap;
+  wahgpu_counts counts; /* out */
+} wahgpu_result;
+int wahgpu_build_batch(wahgpu_ctx* ctx, const uint32_t* const* values, const uint64_t* n,
+                       const uint32_t* rid_base, uint32_t ncols, wahgpu_result* results);
+
+/* Graph replay for batched device builds.  After wahgpu_batch_reserve(ctx,
+ * max_values, max_cols) the context lays its scratch out for any batch of at
+ * most max_values values (values_len) and max_cols columns; the first such
+ * build captures the launch set into a hipGraph, later ones with the same
+ * buffers upload their descriptors and replay it.  (0, 0) turns it off. */
+int wahgpu_batch_reserve(wahgpu_ctx* ctx, uint64_t max_values, uint32_t max_cols);
+/* Number of graph captures and replays so far (for tests and tuning). */
+int wahgpu_graph_stats(const wahgpu_ctx* ctx, uint64_t* captures, uint64_t* replays);
 
 /* ---- stages (device pointers; each synchronizes when it returns a count) */
 /* include/sort_rids_by_value.cl:24-30 + include/radix.cl:65-233: stable sort

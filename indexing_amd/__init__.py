@@ -184,6 +184,83 @@ class Context:
         raw = counts_tensor.cpu().numpy().view(np.uint8).tobytes()
         return N.Counts.from_buffer_copy(raw[:ctypes.sizeof(N.Counts)])
 
+    @staticmethod
+    def read_counts_batch(counts_tensor, ncols):
+        raw = counts_tensor.cpu().numpy().view(np.uint8).tobytes()
+        sz = ctypes.sizeof(N.Counts)
+        return [N.Counts.from_buffer_copy(raw[i * sz:(i + 1) * sz]) for i in range(ncols)]
+
+    # -- batched build --------------------------------------------------------
+    def build_batch(self, columns, rid_bases=None):
+        """Index many host columns (uint32 arrays) in one launch set; returns
+        one Index per column (wahgpu_build_batch)."""
+        cols = [_as_u32(c) for c in columns]
+        m = len(cols)
+        if m == 0:
+            return []
+        vals = (ctypes.c_void_p * m)(*[c.ctypes.data_as(ctypes.c_void_p) for c in cols])
+        ns = (ctypes.c_uint64 * m)(*[c.size for c in cols])
+        rb = (ctypes.c_uint32 * m)(*[int(r) for r in rid_bases]) if rid_bases is not None else None
+        res = (N.Result * m)()
+        bufs = []
+        for i, c in enumerate(cols):
+            cap = max(c.size, 1)
+            b = (np.empty(cap, np.uint32), np.empty(cap, np.uint32), np.empty(cap, np.uint32),
+                 np.empty(2 * cap, np.uint32))
+            bufs.append(b)
+            r = res[i]
+            r.keys, r.offsets, r.lengths = (x.ctypes.data for x in b[:3])
+            r.keys_cap = cap
+            r.words = b[3].ctypes.data
+            r.words_cap = 2 * cap
+        self._check(self._lib.wahgpu_build_batch(
+            self._h, ctypes.cast(vals, ctypes.POINTER(ctypes.c_void_p)), ns,
+            ctypes.cast(rb, N.c_u32p) if rb is not None else None, m, res))
+        out = []
+        for i in range(m):
+            c = res[i].counts
+            K, L = int(c.keycnt), int(c.length)
+            k, o, ln, w = bufs[i]
+            out.append(Index(k[:K].copy(), o[:K].copy(), ln[:K].copy(), w[:L].copy(), int(c.k),
+                             L, K, int(c.bound), int(c.passes)))
+        return out
+
+    def build_batch_device(self, d_values, columns, out=None):
+        """Batched build on one device tensor holding every column.  columns is
+        a list of (offset, n[, rid_base]) ascending and not overlapping.
+        Returns (keys, offsets, lengths, words, counts): column c's keys sit at
+        [offset, offset + n), its words at [2 offset, 2 offset + 2 n), its
+        counts at counts[c] (use read_counts_batch).  Asynchronous."""
+        torch = _torch()
+        m = len(columns)
+        arr = (N.Column * max(m, 1))()
+        for i, col in enumerate(columns):
+            arr[i].offset, arr[i].n = int(col[0]), int(col[1])
+            arr[i].rid_base = int(col[2]) if len(col) > 2 else 0
+            arr[i].reserved = 0
+        total = int(d_values.numel())
+        if out is None:
+            cap = max(total, 1)
+            out = (self._empty(cap), self._empty(cap), self._empty(cap), self._empty(2 * cap),
+                   torch.zeros(max(m, 1) * ctypes.sizeof(N.Counts) // 4, dtype=torch.int32,
+                               device=d_values.device))
+        keys, offs, lens, words, counts = out
+        self._bind_stream()
+        self._check(self._lib.wahgpu_build_batch_device(
+            self._h, _ptr(d_values), total, arr, m, _ptr(keys), _ptr(offs), _ptr(lens),
+            _ptr(words), _ptr(counts)))
+        return out
+
+    def batch_reserve(self, max_values, max_cols):
+        """Lay scratch out for batches up to these sizes and replay them from
+        one captured hipGraph ((0, 0) turns it off)."""
+        self._check(self._lib.wahgpu_batch_reserve(self._h, int(max_values), int(max_cols)))
+
+    def graph_stats(self):
+        cap, rep = ctypes.c_uint64(0), ctypes.c_uint64(0)
+        self._check(self._lib.wahgpu_graph_stats(self._h, ctypes.byref(cap), ctypes.byref(rep)))
+        return int(cap.value), int(rep.value)
+
     # -- stages (device tensors) -------------------------------------------
     def sort_rids_by_value(self, values, rid_base=0):
         n = values.numel()

@@ -43,6 +43,16 @@ class GenParams(ctypes.Structure):
                 ("seed", ctypes.c_uint64)]
 
 
+class Column(ctypes.Structure):
+    _fields_ = [("offset", ctypes.c_uint64), ("n", ctypes.c_uint64),
+                ("rid_base", ctypes.c_uint32), ("reserved", ctypes.c_uint32)]
+
+
+class Result(ctypes.Structure):
+    _fields_ = [("keys", vp), ("offsets", vp), ("lengths", vp), ("keys_cap", ctypes.c_uint64),
+                ("words", vp), ("words_cap", ctypes.c_uint64), ("counts", Counts)]
+
+
 # name -> (restype, argtypes); every function the header declares.
 SIGNATURES = {
     "wahgpu_device_count": (ctypes.c_int, [ctypes.POINTER(ctypes.c_int)]),
@@ -68,6 +78,13 @@ SIGNATURES = {
                                            ctypes.c_uint64, vp, ctypes.c_uint64,
                                            ctypes.POINTER(Counts),
                                            ctypes.POINTER(PhaseTimes)]),
+    "wahgpu_build_batch_device": (ctypes.c_int, [vp, vp, ctypes.c_uint64,
+                                                 ctypes.POINTER(Column), ctypes.c_uint32,
+                                                 vp, vp, vp, vp, vp]),
+    "wahgpu_build_batch": (ctypes.c_int, [vp, ctypes.POINTER(vp), c_u64p, c_u32p,
+                                          ctypes.c_uint32, ctypes.POINTER(Result)]),
+    "wahgpu_batch_reserve": (ctypes.c_int, [vp, ctypes.c_uint64, ctypes.c_uint32]),
+    "wahgpu_graph_stats": (ctypes.c_int, [vp, c_u64p, c_u64p]),
     "wahgpu_sort_rids_by_value": (ctypes.c_int, [vp, vp, ctypes.c_uint64, ctypes.c_uint32,
                                                  vp, vp]),
     "wahgpu_produce_chunk_id_literals": (ctypes.c_int, [vp, vp, ctypes.c_uint64, vp, vp]),

@@ -130,6 +130,21 @@ __device__ __forceinline__ uint64_t wave_sum64(uint64_t v) {
   return v;
 }
 
+// Index of the last entry of a nondecreasing table `start` (count entries,
+// `stride` words apart) with start <= x: the segment that holds x.  Empty
+// segments share their start with the next one, which this skips.  Uniform
+// arguments keep the search in scalar loads.
+__device__ __forceinline__ uint32_t find_segment(const uint32_t* start, uint32_t stride,
+                                                 uint32_t count, uint32_t x) {
+  uint32_t lo = 0, hi = count;  // answer in [lo, hi)
+  while (hi - lo > 1) {
+    const uint32_t mid = (lo + hi) / 2;
+    if (start[(uint64_t)mid * stride] <= x) lo = mid;
+    else hi = mid;
+  }
+  return lo;
+}
+
 // ---- associative operators on 62-bit look-back payloads -------------------
 // combine(a, b): a covers the earlier elements, b the later ones.
 
@@ -229,17 +244,20 @@ __device__ __forceinline__ uint64_t wait_status(uint64_t* p, uint32_t* fault) {
 #ifndef WAH_LOOKAHEAD
 #define WAH_LOOKAHEAD 4
 #endif
+// Tiles below `first` (another column's) count as an empty inclusive prefix.
 __device__ __forceinline__ uint32_t lookback_digit(uint32_t* status, uint32_t tile,
-                                                   uint32_t stride, uint32_t* fault) {
+                                                   uint32_t first, uint32_t stride,
+                                                   uint32_t* fault) {
   constexpr int LA = WAH_LOOKAHEAD;
   uint32_t acc = 0;
+  const int64_t lo = first;
   int64_t at = (int64_t)tile - 1;  // newest tile not yet summed
   uint32_t spins = 0;
-  while (at >= 0) {
+  while (at >= lo) {
     uint32_t s[LA];
 #pragma unroll
     for (int k = 0; k < LA; ++k)
-      s[k] = at - k >= 0 ? ld_agent(&status[(uint64_t)(at - k) * stride]) : (kFlagPrefix << 30);
+      s[k] = at - k >= lo ? ld_agent(&status[(uint64_t)(at - k) * stride]) : (kFlagPrefix << 30);
     int k = 0;
     bool done = false;
 #pragma unroll
@@ -292,19 +310,22 @@ __device__ uint64_t lookback64(uint64_t* status, uint32_t tile, uint64_t agg, ui
 // Look-back for commutative sums, run by a whole wave: each round reads the
 // statuses of the 64 preceding tiles at once and stops at the nearest
 // inclusive prefix.  Returns the exclusive prefix in every lane.
-__device__ uint64_t lookback_wave_sum(uint64_t* status, uint32_t tile, uint64_t agg,
-                                      uint32_t* fault) {
+// Tile `first` opens a segment (a column): it publishes its prefix at once and
+// nothing below it is read.
+__device__ uint64_t lookback_wave_sum(uint64_t* status, uint32_t tile, uint32_t first,
+                                      uint64_t agg, uint32_t* fault) {
   constexpr uint64_t kMask = (1ull << 62) - 1;
   const uint32_t lane = lane_id();
   if (lane == 0)
-    st_agent(&status[tile], ((uint64_t)(tile == 0 ? kFlagPrefix : kFlagAgg) << 62) | (agg & kMask));
-  if (tile == 0) return 0;
+    st_agent(&status[tile], ((uint64_t)(tile == first ? kFlagPrefix : kFlagAgg) << 62) | (agg & kMask));
+  if (tile == first) return 0;
   uint64_t acc = 0;
   int64_t base = (int64_t)tile - 1;
+  const int64_t lo = first;
   uint32_t spins = 0;
   for (;;) {
     const int64_t idx = base - (int64_t)lane;
-    const uint64_t s = idx >= 0 ? ld_agent(&status[idx]) : ((uint64_t)kFlagPrefix << 62);
+    const uint64_t s = idx >= lo ? ld_agent(&status[idx]) : ((uint64_t)kFlagPrefix << 62);
     const uint32_t f = (uint32_t)(s >> 62);
     const uint64_t pmask = ballot(f == kFlagPrefix);
     const uint64_t rmask = ballot(f != 0);

@@ -45,45 +45,92 @@ constexpr uint32_t kBufValues = 0;  // the input column, rids implicit (rid_base
 constexpr uint32_t kBufA = 1;
 constexpr uint32_t kBufB = 2;
 
-struct PassInfo {
-  uint32_t active;      // digit not constant over the column
-  uint32_t shift;       // 8 * digit position
-  uint32_t src, dst;    // kBuf*
-  uint32_t status_sel;  // which of the two status arrays this pass uses
-  uint32_t ordinal;     // index among active passes
-};
-
-constexpr uint32_t kHistCopies = 32;        // striped copies of the global histogram
-constexpr uint32_t kHistCopyStride = 1088;  // words between copies (4 KiB + 256 B)
 constexpr uint32_t kLaneWords = 8 * 64;     // tile-counter lanes of one kernel
+constexpr uint32_t kHistChunk = 4096;       // values per histogram work item
 
-struct SortState {
-  // --- zeroed before every build (kSortHeaderWords words) ---
-  uint32_t hist[kHistCopies][kHistCopyStride];  // [copy][digit * 256 + bin]
-  uint32_t tiles[kCounters][kLaneWords];        // tile-counter lanes per kernel
-  uint32_t maxval;
-  // --- written by k_plan ---
-  uint32_t digit_offset[4][256];  // exclusive prefix of the digit histogram
-  PassInfo pass[4];
-  uint32_t num_active;
-  uint32_t final_buf;             // buffer holding the sorted pairs
-};
-constexpr uint64_t kSortHeaderWords =
-    (uint64_t)kHistCopies * kHistCopyStride + (uint64_t)kCounters * kLaneWords + 1;
-
-struct BuildArgs {
-  const uint32_t* values;
+// One column of a build.  Its values are values[off, off + n) of the
+// concatenated input; its sorted pairs, keys, offsets and lengths use the same
+// range of their buffers, its words [2 off, 2 off + 2 n).  Tiles and histogram
+// chunks never straddle two columns; the *0 fields are the column's first
+// global tile / chunk (prefix sums over the preceding columns).
+struct Column {
+  uint64_t off;
   uint64_t n;
   uint32_t rid_base;
-  uint32_t *keysA, *ridsA, *keysB, *ridsB;
-  SortState* st;
-  uint32_t* sort_status;   // sort_status_words(n)
-  uint64_t* tail_status;   // tail_status_words(n)
-  uint32_t *out_keys, *out_offsets, *out_lengths, *out_words;
-  uint32_t *sorted_keys, *sorted_rids;  // stage API sort output
-  wahgpu_counts_dev* counts;
-  uint32_t max_resident;   // resident 256-thread blocks on the device
+  uint32_t sort_tile0;
+  uint32_t tail_tile0;
+  uint32_t hist_chunk0;
 };
+static_assert(sizeof(Column) == 32, "descriptor layout");
+
+struct Dims {
+  uint32_t ncols, sort_tiles, tail_tiles, hist_chunks;
+};
+// Device descriptor: dims, padding, then ncols Columns.
+struct DescHead {
+  Dims dims;
+  uint32_t pad[4];
+};
+__host__ __device__ inline const Column* desc_cols(const DescHead* d) {
+  return reinterpret_cast<const Column*>(d + 1);
+}
+__host__ __device__ inline Column* desc_cols(DescHead* d) {
+  return reinterpret_cast<Column*>(d + 1);
+}
+
+// Build-wide state, zeroed by k_begin.
+struct Header {
+  uint32_t tiles[kCounters][kLaneWords];  // tile-counter lanes per kernel
+  uint32_t fault;                         // nonzero after a failed wait / bad index
+  uint32_t pad[63];
+};
+
+// Per-column state.  hist and maxval are zeroed by k_begin; the rest is
+// written by k_plan.
+struct ColState {
+  uint32_t hist[4][256];          // histogram of each 8-bit digit
+  uint32_t maxval;
+  uint32_t num_active;            // radix passes this column needs
+  uint32_t final_buf;             // kBuf* holding its sorted pairs
+  uint32_t shift[4];              // digit shift of its a-th active pass
+  uint32_t pad[57];
+  uint32_t digit_offset[4][256];  // exclusive prefix of each digit histogram
+};
+constexpr uint32_t kColZeroWords = 4 * 256 + 1;
+
+struct BuildArgs {
+  const uint32_t* values;            // concatenated input
+  uint32_t *keysA, *ridsA, *keysB, *ridsB;
+  DescHead* desc;                    // device descriptor
+  Header* hdr;
+  ColState* cs;                      // cols_cap entries
+  uint32_t* sort_status;             // 2 arrays of sort_status_stride words
+  uint64_t sort_status_stride;
+  uint64_t* tail_status;             // tail_tiles_cap words
+  uint32_t *out_keys, *out_offsets, *out_lengths, *out_words;
+  uint32_t *sorted_keys, *sorted_rids;  // stage API sort output (one column)
+  wahgpu_counts_dev* counts;         // per column
+  uint32_t max_resident;             // resident 256-thread blocks on the device
+  // Upper bounds the grids are sized for (a captured graph replays any batch
+  // within them; the kernels read the actual sizes from the descriptor).
+  uint32_t cols_cap, sort_tiles_cap, tail_tiles_cap, hist_chunks_cap;
+  uint64_t n_cap;
+  // Single column: k_begin writes the descriptor from these (no host copy).
+  bool inline_desc;
+  Dims one_dims;
+  Column one;
+};
+
+// Tile / chunk counts of a column of n values.
+__host__ __device__ inline uint32_t sort_tiles_of(uint64_t n) {
+  return (uint32_t)((n + kSortTile - 1) / kSortTile);
+}
+__host__ __device__ inline uint32_t tail_tiles_of(uint64_t n) {
+  return (uint32_t)((n + kTailTile - 1) / kTailTile);
+}
+__host__ __device__ inline uint32_t hist_chunks_of(uint64_t n) {
+  return (uint32_t)((n + kHistChunk - 1) / kHistChunk);
+}
 
 struct ScanArgs {
   uint64_t* status;        // scan_status_words(n), zeroed
@@ -92,14 +139,14 @@ struct ScanArgs {
   uint32_t max_resident;
 };
 
-uint64_t sort_status_words(uint64_t n);
-uint64_t tail_status_words(uint64_t n);
 uint64_t scan_status_words(uint64_t n);
 
 // Optional hook called between kernel groups of the fused build (timing).
 struct LaunchHook {
   virtual void mark(hipStream_t s) = 0;
 };
+// The fused build: k_begin, k_histogram, k_plan, 4 radix launches (the a-th
+// launch runs each column's a-th active pass), k_tail, k_finish.
 void launch_sort(const BuildArgs& a, hipStream_t s, LaunchHook* hook = nullptr);
 void launch_tail(const BuildArgs& a, hipStream_t s, LaunchHook* hook = nullptr);
 void launch_sort_finalize(const BuildArgs& a, hipStream_t s);

@@ -30,8 +30,31 @@ __global__ __launch_bounds__(kBlock) void k_fill32(uint32_t* __restrict__ p, uin
     p[i] = v;
 }
 
+// Zeroes the build header and the per-column histograms; a single-column
+// build also gets its descriptor written here (no host-to-device copy).
+__global__ __launch_bounds__(kBlock) void k_begin(Header* __restrict__ hdr,
+                                                  ColState* __restrict__ cs, uint32_t cols_cap,
+                                                  DescHead* __restrict__ desc, int inline_desc,
+                                                  Dims one_dims, Column one) {
+  const uint64_t gt = (uint64_t)blockIdx.x * kBlock + threadIdx.x;
+  const uint64_t gs = (uint64_t)gridDim.x * kBlock;
+  uint32_t* h = reinterpret_cast<uint32_t*>(hdr);
+  for (uint64_t i = gt; i < sizeof(Header) / 4; i += gs) h[i] = 0;
+  const uint64_t zw = (uint64_t)cols_cap * kColZeroWords;
+  for (uint64_t i = gt; i < zw; i += gs) {
+    const uint64_t c = i / kColZeroWords, w = i % kColZeroWords;
+    reinterpret_cast<uint32_t*>(&cs[c])[w] = 0;
+  }
+  if (inline_desc && gt == 0) {
+    desc->dims = one_dims;
+    desc_cols(desc)[0] = one;
+  }
+}
+
 // ---------------------------------------------------------------------------
-// Histogram of all four 8-bit digits (reads the values once).
+// Histogram of all four 8-bit digits per column (reads the values once).
+// Block b takes a contiguous range of 4096-value chunks, accumulates in LDS
+// and folds into the column's histogram when the column changes.
 // ---------------------------------------------------------------------------
 __device__ __forceinline__ void hist_add(uint32_t* h, uint32_t bin, bool valid) {
   // A wave whose valid lanes all share the bin adds once (the high digits of
@@ -46,37 +69,69 @@ __device__ __forceinline__ void hist_add(uint32_t* h, uint32_t bin, bool valid) 
   }
 }
 
+__device__ __forceinline__ void hist_flush(uint32_t* h, uint32_t* s_max, uint32_t& vmax,
+                                           ColState* col) {
+  const int t = threadIdx.x;
+  for (int d = 32; d >= 1; d >>= 1) {
+    const uint32_t o = __shfl_xor(vmax, d, kWave);
+    vmax = o > vmax ? o : vmax;
+  }
+  if ((t % kWave) == 0) s_max[t / kWave] = vmax;
+  __syncthreads();
+  for (int i = t; i < 1024; i += kBlock) {
+    if (h[i]) atomicAdd(&col->hist[0][i], h[i]);
+    h[i] = 0;
+  }
+  if (t == 0) {
+    uint32_t m = 0;
+    for (int i = 0; i < kWaves; ++i) m = s_max[i] > m ? s_max[i] : m;
+    atomicMax(&col->maxval, m);
+  }
+  vmax = 0;
+  __syncthreads();
+}
+
 __global__ __launch_bounds__(kBlock) void k_histogram(
-    const uint32_t* __restrict__ values, uint64_t n, int vec4, SortState* __restrict__ st,
-    uint32_t* __restrict__ status0, uint64_t status0_words,
-    uint64_t* __restrict__ tail_status, uint64_t tail_words,
-    wahgpu_counts_dev* __restrict__ counts) {
+    const uint32_t* __restrict__ values, const DescHead* __restrict__ desc,
+    ColState* __restrict__ cs, uint32_t* __restrict__ status0,
+    uint64_t* __restrict__ tail_status) {
   __shared__ uint32_t h[4 * 256];
   __shared__ uint32_t s_max[kWaves];
   const int t = threadIdx.x;
+  const Dims dims = desc->dims;
+  const Column* cols = desc_cols(desc);
   for (int i = t; i < 1024; i += kBlock) h[i] = 0;
-  if (blockIdx.x == 0 && t == 0) {
-    counts->k = 0; counts->length = 0; counts->keycnt = 0;
-    counts->bound = 0; counts->passes = 0; counts->fault = 0; counts->reserved = 0;
-  }
   // zero look-back state used later in this build
   const uint64_t gthreads = (uint64_t)gridDim.x * kBlock;
   const uint64_t gt = (uint64_t)blockIdx.x * kBlock + t;
-  for (uint64_t i = gt; i < status0_words; i += gthreads) status0[i] = 0;
-  for (uint64_t i = gt; i < tail_words; i += gthreads) tail_status[i] = 0;
+  const uint64_t s0 = (uint64_t)dims.sort_tiles * 256;
+  for (uint64_t i = gt; i < s0; i += gthreads) status0[i] = 0;
+  for (uint64_t i = gt; i < dims.tail_tiles; i += gthreads) tail_status[i] = 0;
   __syncthreads();
 
+  const uint32_t per = (dims.hist_chunks + gridDim.x - 1) / gridDim.x;
+  uint32_t c0 = blockIdx.x * per;
+  const uint32_t c1 = c0 + per < dims.hist_chunks ? c0 + per : dims.hist_chunks;
+  if (c0 >= c1) return;
+  uint32_t col = find_segment(&cols[0].hist_chunk0, sizeof(Column) / 4, dims.ncols, c0);
+  uint32_t col_end = col + 1 < dims.ncols ? cols[col + 1].hist_chunk0 : dims.hist_chunks;
   uint32_t vmax = 0;
-  // Grid-stride over chunks of 4096 values, four 16-byte loads per thread.
-  constexpr uint64_t kChunk = 4 * 4 * kBlock;
-  uint64_t scalar_lo = 0;
-  if (vec4) {
-    const uint4* v4 = reinterpret_cast<const uint4*>(values);
-    const uint64_t full = n / kChunk;
-    for (uint64_t c = blockIdx.x; c < full; c += gridDim.x) {
+  for (uint32_t c = c0; c < c1; ++c) {
+    if (c >= col_end) {  // next column (block-uniform)
+      hist_flush(h, s_max, vmax, &cs[col]);
+      col = find_segment(&cols[0].hist_chunk0, sizeof(Column) / 4, dims.ncols, c);
+      col_end = col + 1 < dims.ncols ? cols[col + 1].hist_chunk0 : dims.hist_chunks;
+    }
+    const Column cd = cols[col];
+    const uint64_t lo = (uint64_t)(c - cd.hist_chunk0) * kHistChunk;
+    const uint32_t cnt = cd.n - lo < kHistChunk ? (uint32_t)(cd.n - lo) : kHistChunk;
+    const uint32_t* src = values + cd.off + lo;
+    if (cnt == kHistChunk && ((uintptr_t)src % 16) == 0) {
+      // four 16-byte loads per thread
+      const uint4* v4 = reinterpret_cast<const uint4*>(src);
       uint4 q[4];
 #pragma unroll
-      for (int u = 0; u < 4; ++u) q[u] = v4[c * (kChunk / 4) + u * kBlock + t];
+      for (int u = 0; u < 4; ++u) q[u] = v4[u * kBlock + t];
 #pragma unroll
       for (int u = 0; u < 4; ++u) {
         const uint32_t e[4] = {q[u].x, q[u].y, q[u].z, q[u].w};
@@ -87,94 +142,78 @@ __global__ __launch_bounds__(kBlock) void k_histogram(
           for (int p = 0; p < 4; ++p) hist_add(h, p * 256 + ((e[j] >> (8 * p)) & 255u), true);
         }
       }
-    }
-    scalar_lo = full * kChunk;
-  }
-  // remainder (or unaligned input): one value per thread per step
-  for (uint64_t b = scalar_lo + (uint64_t)blockIdx.x * kBlock; b < n;
-       b += (uint64_t)gridDim.x * kBlock) {
-    const uint64_t i = b + t;
-    const bool valid = i < n;
-    const uint32_t v = valid ? values[i] : 0;
-    vmax = valid && v > vmax ? v : vmax;
+    } else {
+      for (uint32_t b = 0; b < cnt; b += kBlock) {
+        const uint32_t i = b + t;
+        const bool valid = i < cnt;
+        const uint32_t v = valid ? src[i] : 0;
+        vmax = valid && v > vmax ? v : vmax;
 #pragma unroll
-    for (int p = 0; p < 4; ++p) hist_add(h, p * 256 + ((v >> (8 * p)) & 255u), valid);
+        for (int p = 0; p < 4; ++p) hist_add(h, p * 256 + ((v >> (8 * p)) & 255u), valid);
+      }
+    }
   }
-  for (int d = 32; d >= 1; d >>= 1) {
-    const uint32_t o = __shfl_xor(vmax, d, kWave);
-    vmax = o > vmax ? o : vmax;
-  }
-  if ((t % kWave) == 0) s_max[t / kWave] = vmax;
-  __syncthreads();
-  // fold into one striped copy of the column histogram (zeroed by the
-  // previous launch); copies sit on different memory channels
-  uint32_t* copy = st->hist[blockIdx.x % kHistCopies];
-  for (int i = t; i < 1024; i += kBlock)
-    if (h[i]) atomicAdd(&copy[i], h[i]);
-  if (t == 0) {
-    uint32_t m = 0;
-    for (int i = 0; i < kWaves; ++i) m = s_max[i] > m ? s_max[i] : m;
-    atomicMax(&st->maxval, m);
-  }
+  hist_flush(h, s_max, vmax, &cs[col]);
 }
 
-// One block, after the histogram: digit offsets and the pass plan.  A kernel
-// of its own so the histogram blocks need no device-scope fences (on gfx950
+// After the histogram, per column: digit offsets and the pass plan (digits
+// that are the same for every value of the column are skipped).  A kernel of
+// its own so the histogram blocks need no device-scope fences (on gfx950
 // each one writes back and invalidates the L2).
-__global__ __launch_bounds__(kBlock) void k_plan(SortState* __restrict__ st, uint64_t n,
+__global__ __launch_bounds__(kBlock) void k_plan(const DescHead* __restrict__ desc,
+                                                 ColState* __restrict__ cs,
                                                  wahgpu_counts_dev* __restrict__ counts) {
   __shared__ uint64_t s_scan[kWaves];
   __shared__ int s_trivial[4];
   const int t = threadIdx.x;
-  if (t < 4) s_trivial[t] = 0;
-  __syncthreads();
-  for (int p = 0; p < 4; ++p) {
-    uint32_t c = 0;
-    for (uint32_t k = 0; k < kHistCopies; ++k) c += st->hist[k][p * 256 + t];
-    uint64_t total;
-    const uint64_t ex = block_exclusive<OpAdd>(c, s_scan, &total);
-    st->digit_offset[p][t] = (uint32_t)ex;
-    if ((uint64_t)c == n) s_trivial[p] = 1;  // every value has digit t
-  }
-  __syncthreads();
-  if (t == 0) {
-    uint32_t a = 0;
+  const uint32_t ncols = desc->dims.ncols;
+  for (uint32_t c = blockIdx.x; c < ncols; c += gridDim.x) {
+    const uint64_t n = desc_cols(desc)[c].n;
+    ColState* col = &cs[c];
+    if (t < 4) s_trivial[t] = 0;
+    __syncthreads();
     for (int p = 0; p < 4; ++p) {
-      PassInfo pi;
-      pi.active = (n > 0 && !s_trivial[p]) ? 1u : 0u;
-      pi.shift = 8u * p;
-      pi.dst = (a % 2 == 0) ? kBufA : kBufB;
-      pi.src = (a == 0) ? kBufValues : ((a - 1) % 2 == 0 ? kBufA : kBufB);
-      pi.status_sel = a % 2;
-      pi.ordinal = a;
-      st->pass[p] = pi;
-      if (pi.active) ++a;
+      const uint32_t h = col->hist[p][t];
+      uint64_t total;
+      const uint64_t ex = block_exclusive<OpAdd>(h, s_scan, &total);
+      col->digit_offset[p][t] = (uint32_t)ex;
+      if ((uint64_t)h == n) s_trivial[p] = 1;  // every value has digit t
     }
-    st->num_active = a;
-    st->final_buf = (a == 0) ? kBufValues : ((a - 1) % 2 == 0 ? kBufA : kBufB);
-    counts->bound = st->maxval;
-    counts->passes = a;
+    __syncthreads();
+    if (t == 0) {
+      uint32_t a = 0;
+      for (int p = 0; p < 4; ++p)
+        if (n > 0 && !s_trivial[p]) col->shift[a++] = 8u * p;
+      for (uint32_t q = a; q < 4; ++q) col->shift[q] = 0;
+      col->num_active = a;
+      col->final_buf = (a == 0) ? kBufValues : ((a - 1) % 2 == 0 ? kBufA : kBufB);
+      wahgpu_counts_dev z;
+      z.k = 0; z.length = 0; z.keycnt = 0;
+      z.bound = col->maxval; z.passes = a; z.fault = 0; z.reserved = 0;
+      counts[c] = z;
+    }
+    __syncthreads();
   }
 }
 
 // ---------------------------------------------------------------------------
-// One stable LSD radix pass over 8-bit digit p (Onesweep-style).
-// Tile = THREADS x ITEMS keys; wave w ranks keys [w*64*ITEMS, +64*ITEMS) of
-// the tile in (item, lane) order, which is the input order, so equal digits
-// keep their relative order (stability).  Keys and then rids are staged in
-// one LDS buffer in digit order and written out in runs per digit.
+// Launch `a` of the stable LSD radix sort runs each column's a-th active pass
+// (Onesweep-style): launch 0 reads the values (rids implicit) into buffer A,
+// then A -> B, B -> A, A -> B.  Tile = THREADS x ITEMS keys of one column;
+// wave w ranks keys [w*64*ITEMS, +64*ITEMS) of the tile in (item, lane)
+// order, which is the input order, so equal digits keep their relative order
+// (stability).  Keys and then rids are staged in one LDS buffer in digit
+// order and written out in runs per digit.
 // ---------------------------------------------------------------------------
 template <int THREADS, int ITEMS>
 __global__ __launch_bounds__(THREADS, WAH_SORT_WAVES_PER_EU) void k_onesweep(
-    int p, const uint32_t* __restrict__ values, uint32_t* keysA, uint32_t* ridsA,
-    uint32_t* keysB, uint32_t* ridsB, uint64_t n, uint32_t rid_base,
-    SortState* __restrict__ st, uint32_t* status_base, uint64_t status_stride,
-    uint32_t num_tiles, uint32_t lanes, uint32_t* fault) {
+    uint32_t a, const uint32_t* __restrict__ values, uint32_t* keysA, uint32_t* ridsA,
+    uint32_t* keysB, uint32_t* ridsB, const DescHead* __restrict__ desc,
+    const ColState* __restrict__ cs, Header* __restrict__ hdr, uint32_t* status_base,
+    uint64_t status_stride, uint32_t lanes) {
   constexpr int TILE = THREADS * ITEMS;
   constexpr int WAVES = THREADS / kWave;
   static_assert(THREADS >= 256 && THREADS % 256 == 0, "one thread per digit");
-  const PassInfo info = st->pass[p];
-  if (!info.active) return;
   __shared__ uint32_t s_stage[TILE];
   __shared__ uint8_t s_digit[TILE];
   __shared__ uint32_t s_wcnt[WAVES][256];
@@ -183,24 +222,35 @@ __global__ __launch_bounds__(THREADS, WAH_SORT_WAVES_PER_EU) void k_onesweep(
   __shared__ uint32_t s_dsum[4];
   __shared__ uint32_t s_tile;
 
-  const bool implicit = info.src == kBufValues;
-  const uint32_t* src_keys = implicit ? values : (info.src == kBufA ? keysA : keysB);
-  const uint32_t* src_rids = info.src == kBufA ? ridsA : ridsB;
-  uint32_t* dst_keys = info.dst == kBufA ? keysA : keysB;
-  uint32_t* dst_rids = info.dst == kBufA ? ridsA : ridsB;
-  uint32_t* status = status_base + (uint64_t)info.status_sel * status_stride;
-  uint32_t* status_next = status_base + (uint64_t)(1u - info.status_sel) * status_stride;
-  const uint32_t shift = info.shift;
+  const bool implicit = a == 0;
+  const uint32_t* src_keys = implicit ? values : ((a % 2) ? keysA : keysB);
+  const uint32_t* src_rids = (a % 2) ? ridsA : ridsB;
+  uint32_t* dst_keys = (a % 2) ? keysB : keysA;
+  uint32_t* dst_rids = (a % 2) ? ridsB : ridsA;
+  uint32_t* status = status_base + (uint64_t)(a % 2) * status_stride;
+  uint32_t* status_next = status_base + (uint64_t)(1u - a % 2) * status_stride;
+  uint32_t* fault = &hdr->fault;
+  const Dims dims = desc->dims;
+  const Column* cols = desc_cols(desc);
   const int t = threadIdx.x, w = t / kWave, l = t % kWave;
-  const uint32_t goff = t < 256 ? st->digit_offset[p][t] : 0u;
 
   uint32_t iter = 0;
   for (;;) {
     for (int i = t; i < WAVES * 256; i += THREADS) (&s_wcnt[0][0])[i] = 0;
-    const uint32_t tile = next_tile(st->tiles[p], lanes, &s_tile, iter);
-    if (tile >= num_tiles) break;
-    const uint64_t base = (uint64_t)tile * TILE;
-    if (t < 256) status_next[(uint64_t)tile * 256 + t] = 0;  // next active pass's state
+    const uint32_t tile = __builtin_amdgcn_readfirstlane(next_tile(hdr->tiles[a], lanes, &s_tile, iter));
+    if (tile >= dims.sort_tiles) break;
+    const uint32_t c = find_segment(&cols[0].sort_tile0, sizeof(Column) / 4, dims.ncols, tile);
+    const ColState* col = &cs[c];
+    if (col->num_active <= a) {  // this column is already sorted
+      __syncthreads();           // s_tile is read by every wave before it is rewritten
+      continue;
+    }
+    const Column cd = cols[c];
+    const uint32_t first = cd.sort_tile0;
+    const uint32_t shift = col->shift[a];
+    const uint64_t n = cd.n;
+    const uint64_t base = (uint64_t)(tile - first) * TILE;  // column-local
+    if (t < 256) status_next[(uint64_t)tile * 256 + t] = 0;  // next launch's state
 
     // Keys only for now: the rids are loaded once the keys are staged, so
     // they never occupy registers during ranking and look-back.
@@ -209,7 +259,7 @@ __global__ __launch_bounds__(THREADS, WAH_SORT_WAVES_PER_EU) void k_onesweep(
     const bool full = base + TILE <= n;
     const uint32_t nvalid = full ? (uint32_t)ITEMS * kWave : (uint32_t)(n > wbase ? n - wbase : 0);
     {
-      const uint32_t* kp = src_keys + wbase;  // one base, immediate offsets
+      const uint32_t* kp = src_keys + cd.off + wbase;  // one base, immediate offsets
       if (full) {
 #pragma unroll
         for (int i = 0; i < ITEMS; ++i) key[i] = kp[i * kWave];
@@ -242,21 +292,22 @@ __global__ __launch_bounds__(THREADS, WAH_SORT_WAVES_PER_EU) void k_onesweep(
     // Threads 0..255 own one digit each: wave prefixes, tile total,
     // exclusive scan over digits, look-back.
     if (t < 256) {
+      const uint32_t goff = col->digit_offset[shift / 8][t];
       uint32_t total = 0;
 #pragma unroll
       for (int i = 0; i < WAVES; ++i) {
-        const uint32_t c = s_wcnt[i][t];
+        const uint32_t x = s_wcnt[i][t];
         s_wcnt[i][t] = total;
-        total += c;
+        total += x;
       }
       uint32_t* my = &status[(uint64_t)tile * 256 + t];
-      st_agent(my, ((tile == 0 ? kFlagPrefix : kFlagAgg) << 30) | total);
+      st_agent(my, ((tile == first ? kFlagPrefix : kFlagAgg) << 30) | total);
       const uint32_t inc = wave_incl_add(total);
       if (l == kWave - 1) s_dsum[w] = inc;
       s_excl[t] = inc - total;  // within this wave's 64 digits
       uint32_t prev = 0;
-      if (tile > 0) {
-        prev = lookback_digit(status + t, tile, 256, fault);
+      if (tile > first) {
+        prev = lookback_digit(status + t, tile, first, 256, fault);
         st_agent(my, (kFlagPrefix << 30) | (prev + total));
       }
       s_glob[t] = goff + prev;  // the tile-local offset is subtracted below
@@ -268,7 +319,7 @@ __global__ __launch_bounds__(THREADS, WAH_SORT_WAVES_PER_EU) void k_onesweep(
       for (int i = 0; i < 4; ++i) before += i < w ? s_dsum[i] : 0u;
       const uint32_t ex = s_excl[t] + before;
       s_excl[t] = ex;
-      s_glob[t] -= ex;  // global index = s_glob[d] + local index
+      s_glob[t] -= ex;  // column index = s_glob[d] + local index
     }
     __syncthreads();
 #pragma unroll
@@ -285,20 +336,22 @@ __global__ __launch_bounds__(THREADS, WAH_SORT_WAVES_PER_EU) void k_onesweep(
     uint32_t rid[ITEMS];
     if (implicit) {
 #pragma unroll
-      for (int i = 0; i < ITEMS; ++i) rid[i] = rid_base + (uint32_t)wbase + (uint32_t)(i * kWave);
+      for (int i = 0; i < ITEMS; ++i) rid[i] = cd.rid_base + (uint32_t)wbase + (uint32_t)(i * kWave);
     } else {
-      const uint32_t* rp = src_rids + wbase;
+      const uint32_t* rp = src_rids + cd.off + wbase;
 #pragma unroll
       for (int i = 0; i < ITEMS; ++i) rid[i] = (uint32_t)(i * kWave) < nvalid ? rp[i * kWave] : 0u;
     }
     __syncthreads();
     const uint32_t tile_n = (n - base) < (uint64_t)TILE ? (uint32_t)(n - base) : (uint32_t)TILE;
+    uint32_t* dk = dst_keys + cd.off;
+    uint32_t* dr = dst_rids + cd.off;
 #pragma unroll
     for (int j = 0; j < ITEMS; ++j) {
       const uint32_t q = (uint32_t)j * THREADS + t;
       if (q < tile_n) {
         const uint32_t g = s_glob[s_digit[q]] + q;
-        if (g < n) dst_keys[g] = s_stage[q];
+        if (g < n) dk[g] = s_stage[q];
         else atomicOr(fault, 2u);
       }
     }
@@ -312,27 +365,29 @@ __global__ __launch_bounds__(THREADS, WAH_SORT_WAVES_PER_EU) void k_onesweep(
       const uint32_t q = (uint32_t)j * THREADS + t;
       if (q < tile_n) {
         const uint32_t g = s_glob[s_digit[q]] + q;
-        if (g < n) dst_rids[g] = s_stage[q];
+        if (g < n) dr[g] = s_stage[q];
       }
     }
     __syncthreads();
   }
 }
 
-// Selects the sorted arrays the tail reads (after the active passes).
+// Selects the sorted arrays of one column (after its active passes); the
+// pointers address the column's first pair.
 struct SortedView {
   const uint32_t* keys;
   const uint32_t* rids;
   bool implicit;
 };
-__device__ __forceinline__ SortedView sorted_view(const SortState* st, const uint32_t* values,
+__device__ __forceinline__ SortedView sorted_view(const ColState* col, const Column& cd,
+                                                  const uint32_t* values,
                                                   const uint32_t* keysA, const uint32_t* ridsA,
                                                   const uint32_t* keysB, const uint32_t* ridsB) {
   SortedView v;
-  const uint32_t f = st->final_buf;
+  const uint32_t f = col->final_buf;
   v.implicit = f == kBufValues;
-  v.keys = f == kBufValues ? values : (f == kBufA ? keysA : keysB);
-  v.rids = f == kBufA ? ridsA : ridsB;
+  v.keys = (f == kBufValues ? values : (f == kBufA ? keysA : keysB)) + cd.off;
+  v.rids = (f == kBufA ? ridsA : ridsB) + cd.off;
   return v;
 }
 
@@ -349,11 +404,11 @@ __device__ __forceinline__ SortedView sorted_view(const SortState* st, const uin
 // ---------------------------------------------------------------------------
 template <int THREADS, int ITEMS>
 __global__ __launch_bounds__(THREADS) void k_tail(
-    const SortState* __restrict__ st, const uint32_t* values, const uint32_t* keysA,
-    const uint32_t* ridsA, const uint32_t* keysB, const uint32_t* ridsB, uint64_t n,
-    uint32_t rid_base, uint32_t* __restrict__ out_keys, uint32_t* __restrict__ out_offsets,
-    uint32_t* __restrict__ out_words, uint64_t* status, uint32_t num_tiles,
-    uint32_t* tile_counter, uint32_t lanes, wahgpu_counts_dev* __restrict__ counts) {
+    const DescHead* __restrict__ desc, const ColState* __restrict__ cs, Header* __restrict__ hdr,
+    const uint32_t* values, const uint32_t* keysA, const uint32_t* ridsA,
+    const uint32_t* keysB, const uint32_t* ridsB, uint32_t* __restrict__ keys_out,
+    uint32_t* __restrict__ offsets_out, uint32_t* __restrict__ words_out, uint64_t* status,
+    uint32_t lanes, wahgpu_counts_dev* __restrict__ counts) {
   constexpr int TILE = THREADS * ITEMS;
   constexpr int WAVES = THREADS / kWave;
   constexpr int SUB = kWave * ITEMS;     // pairs per wave
@@ -364,21 +419,51 @@ __global__ __launch_bounds__(THREADS) void k_tail(
   __shared__ __align__(16) uint32_t s_buf[2 * SPAN];
   __shared__ uint32_t s_wtot[WAVES];
   __shared__ uint64_t s_prefix;
+  __shared__ uint64_t s_ent[WAVES];
   __shared__ uint32_t s_tile;
   uint32_t* s_k = s_buf;
   uint32_t* s_r = s_buf + SPAN;
   uint32_t* s_words = s_buf;  // reused once the pairs are consumed (2 * TILE words)
-  const SortedView sv = sorted_view(st, values, keysA, ridsA, keysB, ridsB);
+  const Dims dims = desc->dims;
+  const Column* cols = desc_cols(desc);
   const int t = threadIdx.x, w = t / kWave, l = t % kWave;
-  uint64_t entries = 0;
+  uint32_t* fault = &hdr->fault;
+  uint64_t entries = 0;           // entries of column ent_col counted by this thread
+  uint32_t ent_col = 0xFFFFFFFFu;
+  // k of a column: one atomic per block and column
+  auto flush_entries = [&]() {
+    const uint64_t e = wave_sum64(entries);
+    if (l == 0) s_ent[w] = e;
+    __syncthreads();
+    if (t == 0) {
+      uint64_t sum = 0;
+      for (int i = 0; i < WAVES; ++i) sum += s_ent[i];
+      if (sum) atomicAdd((unsigned long long*)&counts[ent_col].k, (unsigned long long)sum);
+    }
+    entries = 0;
+  };
 
   uint32_t iter = 0;
   for (;;) {
-    const uint32_t tile = next_tile(tile_counter, lanes, &s_tile, iter);
-    if (tile >= num_tiles) break;
-    const uint64_t base = (uint64_t)tile * TILE;
+    const uint32_t tile = __builtin_amdgcn_readfirstlane(next_tile(&hdr->tiles[kCounterTail][0], lanes, &s_tile, iter));
+    if (tile >= dims.tail_tiles) break;
+    const uint32_t c = find_segment(&cols[0].tail_tile0, sizeof(Column) / 4, dims.ncols, tile);
+    if (c != ent_col) {  // block-uniform
+      if (ent_col != 0xFFFFFFFFu) flush_entries();
+      ent_col = c;
+    }
+    const Column cd = cols[c];
+    const uint64_t n = cd.n;
+    const uint32_t rid_base = cd.rid_base;
+    const uint32_t first_tile = cd.tail_tile0;
+    const SortedView sv = sorted_view(&cs[c], cd, values, keysA, ridsA, keysB, ridsB);
+    uint32_t* out_keys = keys_out + cd.off;
+    uint32_t* out_offsets = offsets_out + cd.off;
+    uint32_t* out_words = words_out + 2 * cd.off;
+    const uint64_t base = (uint64_t)(tile - first_tile) * TILE;  // column-local
     // stage pairs [base - 1, base + TILE + HALO - 1) with 16-byte loads
-    if (base + TILE <= n) {
+    const bool aligned = (((uintptr_t)(sv.keys + base) | (uintptr_t)(sv.rids + base)) % 16) == 0;
+    if (base + TILE <= n && aligned) {
       const uint4* k4 = reinterpret_cast<const uint4*>(sv.keys + base);
       const uint4* r4 = reinterpret_cast<const uint4*>(sv.rids + base);
       for (int j = t; j < TILE / 4; j += THREADS) {
@@ -464,7 +549,7 @@ __global__ __launch_bounds__(THREADS) void k_tail(
     }
     const uint64_t tile_agg = ((uint64_t)(tot >> 16) << 31) | (tot & 0xFFFFu);
     if (w == 0) {
-      const uint64_t pre = lookback_wave_sum(status, tile, tile_agg, &counts->fault);
+      const uint64_t pre = lookback_wave_sum(status, tile, first_tile, tile_agg, fault);
       if (l == 0) s_prefix = pre;
     }
     __syncthreads();  // every read of the staged pairs is done
@@ -480,7 +565,7 @@ __global__ __launch_bounds__(THREADS) void k_tail(
           out_keys[kidx] = key[i];
           out_offsets[kidx] = Wpre + pos;
         } else {
-          atomicOr(&counts->fault, 2u);
+          atomicOr(fault, 2u);
         }
       }
       if (lit[i]) {
@@ -493,19 +578,46 @@ __global__ __launch_bounds__(THREADS) void k_tail(
     for (uint32_t j = t; j < Wt; j += THREADS) {
       const uint64_t o = (uint64_t)Wpre + j;
       if (o < 2 * n) out_words[o] = s_words[j];
-      else atomicOr(&counts->fault, 2u);
+      else atomicOr(fault, 2u);
     }
-    if (tile == num_tiles - 1 && t == 0) {
-      counts->length = (uint64_t)Wpre + Wt;
-      counts->keycnt = (uint64_t)Kpre + (tot >> 16);
+    if (base + TILE >= n && t == 0) {  // the column's last tile
+      counts[c].length = (uint64_t)Wpre + Wt;
+      counts[c].keycnt = (uint64_t)Kpre + (tot >> 16);
     }
     __syncthreads();
   }
-  // one update of k per wave
-  entries = wave_sum64(entries);
-  if (l == 0 && entries) atomicAdd((unsigned long long*)&counts->k, (unsigned long long)entries);
+  if (ent_col != 0xFFFFFFFFu) flush_entries();
 }
 
+// Per column: lengths[i] = offsets[i+1] - offsets[i], the last key running to
+// L; the build's fault word is copied into every column's counts.
+__global__ __launch_bounds__(kBlock) void k_finish(const DescHead* __restrict__ desc,
+                                                   const Header* __restrict__ hdr,
+                                                   const uint32_t* __restrict__ offsets,
+                                                   uint32_t* __restrict__ lengths,
+                                                   wahgpu_counts_dev* __restrict__ counts) {
+  const uint32_t ncols = desc->dims.ncols;
+  const Column* cols = desc_cols(desc);
+  const uint32_t fault = hdr->fault;
+  const uint64_t gt = (uint64_t)blockIdx.x * kBlock + threadIdx.x;
+  const uint64_t gs = (uint64_t)gridDim.x * kBlock;
+  for (uint32_t c = 0; c < ncols; ++c) {
+    const Column cd = cols[c];
+    const uint64_t kc = counts[c].keycnt;
+    const uint64_t K = kc < cd.n ? kc : cd.n, L = counts[c].length;
+    const uint32_t* o = offsets + cd.off;
+    uint32_t* len = lengths + cd.off;
+    for (uint64_t i = gt; i < K; i += gs) {
+      const uint32_t end = (i + 1 < K) ? o[i + 1] : (uint32_t)L;
+      len[i] = end - o[i];
+    }
+    if (gt == 0 && fault) counts[c].fault = fault;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Staged kernels (one per reference stage) and primitives.
+// ---------------------------------------------------------------------------
 // lengths[i] = offsets[i+1] - offsets[i]; the last key runs to L.
 __global__ __launch_bounds__(kBlock) void k_lengths(const uint32_t* __restrict__ offsets,
                                                     uint32_t* __restrict__ lengths,
@@ -519,20 +631,18 @@ __global__ __launch_bounds__(kBlock) void k_lengths(const uint32_t* __restrict__
   }
 }
 
-// ---------------------------------------------------------------------------
-// Staged kernels (one per reference stage) and primitives.
-// ---------------------------------------------------------------------------
-// Copy the sorted result out of the pass buffers (stage API output).
+// Copy the sorted result of column 0 out of the pass buffers (stage API).
 __global__ __launch_bounds__(kBlock) void k_sort_finalize(
-    const SortState* __restrict__ st, const uint32_t* values, const uint32_t* keysA,
-    const uint32_t* ridsA, const uint32_t* keysB, const uint32_t* ridsB, uint64_t n,
-    uint32_t rid_base, uint32_t* out_keys, uint32_t* out_rids) {
-  const SortedView sv = sorted_view(st, values, keysA, ridsA, keysB, ridsB);
+    const DescHead* __restrict__ desc, const ColState* __restrict__ cs, const uint32_t* values,
+    const uint32_t* keysA, const uint32_t* ridsA, const uint32_t* keysB, const uint32_t* ridsB,
+    uint32_t* out_keys, uint32_t* out_rids) {
+  const Column cd = desc_cols(desc)[0];
+  const SortedView sv = sorted_view(&cs[0], cd, values, keysA, ridsA, keysB, ridsB);
   if (sv.keys == out_keys && !sv.implicit) return;
-  for (uint64_t i = (uint64_t)blockIdx.x * kBlock + threadIdx.x; i < n;
+  for (uint64_t i = (uint64_t)blockIdx.x * kBlock + threadIdx.x; i < cd.n;
        i += (uint64_t)gridDim.x * kBlock) {
     out_keys[i] = sv.keys[i];
-    out_rids[i] = sv.implicit ? rid_base + (uint32_t)i : sv.rids[i];
+    out_rids[i] = sv.implicit ? cd.rid_base + (uint32_t)i : sv.rids[i];
   }
 }
 
@@ -790,69 +900,55 @@ __global__ __launch_bounds__(kBlock) void k_scan(
 // ---------------------------------------------------------------------------
 // Launchers
 // ---------------------------------------------------------------------------
-static void launch_onesweep(const BuildArgs& a, int p, hipStream_t s) {
-  const uint32_t tiles = (uint32_t)((a.n + kSortTile - 1) / kSortTile);
-  const uint32_t cap = WAH_STATIC_TILES ? tiles : a.max_resident * 256 / kSortThreads;
-  uint32_t grid = tiles < cap ? tiles : cap;
-  if (grid == 0) grid = 1;
-  hipLaunchKernelGGL((k_onesweep<kSortThreads, kSortItems>), dim3(grid), dim3(kSortThreads), 0,
-                     s, p, a.values, a.keysA, a.ridsA, a.keysB, a.ridsB, a.n, a.rid_base, a.st,
-                     a.sort_status, (uint64_t)tiles * 256, tiles, tile_lanes(grid),
-                     &a.counts->fault);
+static uint32_t cap_grid(uint32_t work, uint32_t cap) {
+  const uint32_t g = work < cap ? work : cap;
+  return g ? g : 1u;
 }
-
-uint64_t sort_status_words(uint64_t n) {
-  const uint64_t tiles = (n + kSortTile - 1) / kSortTile;
-  return 2 * tiles * 256;
-}
-uint64_t tail_status_words(uint64_t n) { return (n + kTailTile - 1) / kTailTile; }
-uint64_t scan_status_words(uint64_t n) { return (n + kScanTile - 1) / kScanTile; }
 
 void launch_sort(const BuildArgs& a, hipStream_t s, LaunchHook* hook) {
-  uint32_t hb = (uint32_t)((a.n + 16383) / 16384);
-  if (hb > kHistMaxBlocks) hb = kHistMaxBlocks;
-  if (hb == 0) hb = 1;
-  const int vec4 = ((uintptr_t)a.values % 16) == 0;
-  const uint64_t tiles = (a.n + kSortTile - 1) / kSortTile;
-  hipLaunchKernelGGL(k_fill32, dim3(64), dim3(kBlock), 0, s, reinterpret_cast<uint32_t*>(a.st),
-                     kSortHeaderWords, 0u);
-  hipLaunchKernelGGL(k_histogram, dim3(hb), dim3(kBlock), 0, s, a.values, a.n, vec4, a.st,
-                     a.sort_status, tiles * 256, a.tail_status, tail_status_words(a.n),
+  hipLaunchKernelGGL(k_begin, dim3(64), dim3(kBlock), 0, s, a.hdr, a.cs, a.cols_cap, a.desc,
+                     (int)a.inline_desc, a.one_dims, a.one);
+  uint64_t hb = (a.n_cap + 4 * kHistChunk - 1) / (4 * kHistChunk);
+  if (hb > a.hist_chunks_cap) hb = a.hist_chunks_cap;
+  hipLaunchKernelGGL(k_histogram, dim3(cap_grid((uint32_t)hb, kHistMaxBlocks)), dim3(kBlock), 0,
+                     s, a.values, a.desc, a.cs, a.sort_status, a.tail_status);
+  hipLaunchKernelGGL(k_plan, dim3(cap_grid(a.cols_cap, 1024)), dim3(kBlock), 0, s, a.desc, a.cs,
                      a.counts);
-  hipLaunchKernelGGL(k_plan, dim3(1), dim3(kBlock), 0, s, a.st, a.n, a.counts);
   if (hook) hook->mark(s);
-  for (int p = 0; p < 4; ++p) {
-    launch_onesweep(a, p, s);
+  const uint32_t grid =
+      cap_grid(a.sort_tiles_cap, WAH_STATIC_TILES ? a.sort_tiles_cap
+                                                  : a.max_resident * 256 / kSortThreads);
+  for (uint32_t p = 0; p < 4; ++p) {
+    hipLaunchKernelGGL((k_onesweep<kSortThreads, kSortItems>), dim3(grid), dim3(kSortThreads), 0,
+                       s, p, a.values, a.keysA, a.ridsA, a.keysB, a.ridsB, a.desc, a.cs, a.hdr,
+                       a.sort_status, a.sort_status_stride, tile_lanes(grid));
     if (hook) hook->mark(s);
   }
 }
 
 void launch_tail(const BuildArgs& a, hipStream_t s, LaunchHook* hook) {
-  const uint32_t tiles = (uint32_t)((a.n + kTailTile - 1) / kTailTile);
-  const uint32_t cap = WAH_STATIC_TILES ? tiles : a.max_resident * 256 / kTailThreads;
-  const uint32_t grid = tiles < cap ? tiles : cap;
-  if (grid) {
-    hipLaunchKernelGGL((k_tail<kTailThreads, kTailItems>), dim3(grid), dim3(kTailThreads), 0, s,
-                       a.st, a.values, a.keysA, a.ridsA, a.keysB, a.ridsB, a.n, a.rid_base,
-                       a.out_keys, a.out_offsets, a.out_words, a.tail_status, tiles,
-                       a.st->tiles[kCounterTail], tile_lanes(grid), a.counts);
-  }
+  const uint32_t grid =
+      cap_grid(a.tail_tiles_cap, WAH_STATIC_TILES ? a.tail_tiles_cap
+                                                  : a.max_resident * 256 / kTailThreads);
+  hipLaunchKernelGGL((k_tail<kTailThreads, kTailItems>), dim3(grid), dim3(kTailThreads), 0, s,
+                     a.desc, a.cs, a.hdr, a.values, a.keysA, a.ridsA, a.keysB, a.ridsB,
+                     a.out_keys, a.out_offsets, a.out_words, a.tail_status, tile_lanes(grid),
+                     a.counts);
   if (hook) hook->mark(s);
-  uint32_t lg = (uint32_t)((a.n + kBlock - 1) / kBlock);
-  if (lg > 1024) lg = 1024;
-  if (lg == 0) lg = 1;
-  hipLaunchKernelGGL(k_lengths, dim3(lg), dim3(kBlock), 0, s, a.out_offsets, a.out_lengths,
-                     a.counts, a.n);
+  uint64_t lg = (a.n_cap + kBlock - 1) / kBlock;
+  hipLaunchKernelGGL(k_finish, dim3(cap_grid(lg > 1024 ? 1024u : (uint32_t)lg, 1024)),
+                     dim3(kBlock), 0, s, a.desc, a.hdr, a.out_offsets, a.out_lengths, a.counts);
 }
 
 void launch_sort_finalize(const BuildArgs& a, hipStream_t s) {
-  uint32_t g = (uint32_t)((a.n + kBlock - 1) / kBlock);
+  uint64_t g = (a.n_cap + kBlock - 1) / kBlock;
   if (g > 4096) g = 4096;
   if (g == 0) return;
-  hipLaunchKernelGGL(k_sort_finalize, dim3(g), dim3(kBlock), 0, s, a.st, a.values, a.keysA,
-                     a.ridsA, a.keysB, a.ridsB, a.n, a.rid_base, a.sorted_keys,
-                     a.sorted_rids);
+  hipLaunchKernelGGL(k_sort_finalize, dim3((uint32_t)g), dim3(kBlock), 0, s, a.desc, a.cs,
+                     a.values, a.keysA, a.ridsA, a.keysB, a.ridsB, a.sorted_keys, a.sorted_rids);
 }
+
+uint64_t scan_status_words(uint64_t n) { return (n + kScanTile - 1) / kScanTile; }
 
 static uint32_t grid_for(uint64_t n) {
   uint64_t g = (n + kBlock - 1) / kBlock;

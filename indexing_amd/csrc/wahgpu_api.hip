@@ -31,6 +31,16 @@ struct wahgpu_ctx {
   std::vector<hipEvent_t> tev;  // max_calls x (WAHGPU_TIMING_GROUPS + 1)
   uint32_t tmax = 0, tcalls = 0;
   bool timing = false;
+  // batched builds: pinned staging of the descriptor, reusable once its copy ran
+  char* desc_stage = nullptr;
+  uint64_t desc_stage_cap = 0;
+  hipEvent_t desc_copied = nullptr;
+  // graph replay (wahgpu_batch_reserve)
+  uint64_t res_values = 0;
+  uint32_t res_cols = 0;
+  hipGraphExec_t gexec = nullptr;
+  std::vector<uintptr_t> gkey;  // buffers and sizes the graph was captured with
+  uint64_t captures = 0, replays = 0;
 };
 
 namespace {
@@ -88,25 +98,62 @@ T* take(wahgpu_ctx* c, uint64_t count) {
   return p;
 }
 
-// Bytes of scratch a fused build of n values needs (besides outputs).
-uint64_t build_scratch_bytes(uint64_t n) {
+// Sizes a fused build's scratch and grids are laid out for.
+struct Caps {
+  uint64_t n;  // values of the concatenated input (its highest end)
+  uint32_t cols, sort_tiles, tail_tiles, hist_chunks;
+  bool operator==(const Caps& o) const {
+    return n == o.n && cols == o.cols && sort_tiles == o.sort_tiles &&
+           tail_tiles == o.tail_tiles && hist_chunks == o.hist_chunks;
+  }
+  bool covers(const Caps& o) const {
+    return n >= o.n && cols >= o.cols && sort_tiles >= o.sort_tiles &&
+           tail_tiles >= o.tail_tiles && hist_chunks >= o.hist_chunks;
+  }
+};
+Caps caps_one(uint64_t n) {
+  return Caps{n, 1u, wah::sort_tiles_of(n), wah::tail_tiles_of(n), wah::hist_chunks_of(n)};
+}
+
+// Bytes of scratch a fused build needs (besides outputs).
+uint64_t build_scratch_bytes(const Caps& k) {
+  const uint64_t nn = k.n ? k.n : 1;
   uint64_t b = 0;
-  b += 4 * align_up(n * 4);                                  // keysA ridsA keysB ridsB
-  b += align_up(sizeof(wah::SortState));
-  b += align_up(wah::sort_status_words(n) * 4);
-  b += align_up(wah::tail_status_words(n) * 8);
+  b += 4 * align_up(nn * 4);                                       // keysA ridsA keysB ridsB
+  b += align_up(sizeof(wah::DescHead) + (uint64_t)k.cols * sizeof(wah::Column));
+  b += align_up(sizeof(wah::Header));
+  b += align_up((uint64_t)k.cols * sizeof(wah::ColState));
+  b += align_up(2ull * k.sort_tiles * 256 * 4);
+  b += align_up((uint64_t)k.tail_tiles * 8 + 8);
   return b;
 }
 
-void carve_build(wahgpu_ctx* c, BuildArgs* a, uint64_t n) {
-  a->keysA = take<uint32_t>(c, n);
-  a->ridsA = take<uint32_t>(c, n);
-  a->keysB = take<uint32_t>(c, n);
-  a->ridsB = take<uint32_t>(c, n);
-  a->st = take<wah::SortState>(c, 1);
-  a->sort_status = take<uint32_t>(c, wah::sort_status_words(n));
-  a->tail_status = take<uint64_t>(c, wah::tail_status_words(n));
+void carve_build(wahgpu_ctx* c, BuildArgs* a, const Caps& k) {
+  const uint64_t nn = k.n ? k.n : 1;
+  a->keysA = take<uint32_t>(c, nn);
+  a->ridsA = take<uint32_t>(c, nn);
+  a->keysB = take<uint32_t>(c, nn);
+  a->ridsB = take<uint32_t>(c, nn);
+  a->desc = reinterpret_cast<wah::DescHead*>(
+      take<char>(c, sizeof(wah::DescHead) + (uint64_t)k.cols * sizeof(wah::Column)));
+  a->hdr = take<wah::Header>(c, 1);
+  a->cs = take<wah::ColState>(c, k.cols);
+  a->sort_status_stride = (uint64_t)k.sort_tiles * 256;
+  a->sort_status = take<uint32_t>(c, 2 * a->sort_status_stride);
+  a->tail_status = take<uint64_t>(c, (uint64_t)k.tail_tiles + 1);
   a->max_resident = c->max_resident;
+  a->n_cap = k.n;
+  a->cols_cap = k.cols;
+  a->sort_tiles_cap = k.sort_tiles;
+  a->tail_tiles_cap = k.tail_tiles;
+  a->hist_chunks_cap = k.hist_chunks;
+}
+
+// A single column at offset 0: its descriptor travels in k_begin's arguments.
+void set_one(BuildArgs* a, uint64_t n, uint32_t rid_base) {
+  a->inline_desc = true;
+  a->one_dims = wah::Dims{1u, wah::sort_tiles_of(n), wah::tail_tiles_of(n), wah::hist_chunks_of(n)};
+  a->one = wah::Column{0, n, rid_base, 0u, 0u, 0u};
 }
 
 // Scan scratch, zeroed before use: tile-counter lanes (kLaneWords words),
@@ -252,7 +299,7 @@ int wahgpu_create(wahgpu_ctx** out, int device_ordinal, uint64_t arena_bytes) {
   wahgpu_ctx* c = new wahgpu_ctx();
   c->device = device_ordinal;
   if (hipSetDevice(device_ordinal) != hipSuccess ||
-      hipStreamCreateWithFlags(&c->own_stream, hipStreamNonBlocking) != hipSuccess ||
+      hipStreamCreateWithFlags(&c->own_stream, hipStreamDefault) != hipSuccess ||
       hipHostMalloc(reinterpret_cast<void**>(&c->pinned_counts), 64, hipHostMallocDefault) !=
           hipSuccess) {
     (void)hipGetLastError();
@@ -281,6 +328,9 @@ void wahgpu_destroy(wahgpu_ctx* c) {
   (void)hipSetDevice(c->device);
   for (auto e : c->tev) (void)hipEventDestroy(e);
   if (c->stream) (void)hipStreamSynchronize(c->stream);
+  if (c->gexec) (void)hipGraphExecDestroy(c->gexec);
+  if (c->desc_copied) (void)hipEventDestroy(c->desc_copied);
+  if (c->desc_stage) (void)hipHostFree(c->desc_stage);
   if (c->arena) (void)hipFree(c->arena);
   if (c->pinned_counts) (void)hipHostFree(c->pinned_counts);
   if (c->own_stream) (void)hipStreamDestroy(c->own_stream);
@@ -318,10 +368,9 @@ static int build_device_impl(wahgpu_ctx* c, const uint32_t* d_values, uint64_t n
                              uint32_t* d_lengths, uint32_t* d_words, wahgpu_counts* d_counts,
                              Marks* marks) {
   BuildArgs a{};
-  carve_build(c, &a, n);
+  carve_build(c, &a, caps_one(n));
+  set_one(&a, n, rid_base);
   a.values = d_values;
-  a.n = n;
-  a.rid_base = rid_base;
   a.out_keys = d_keys;
   a.out_offsets = d_offsets;
   a.out_lengths = d_lengths;
@@ -385,7 +434,7 @@ int wahgpu_build_device(wahgpu_ctx* c, const uint32_t* d_values, uint64_t n, uin
   int rc = check_n(c, n, rid_base);
   if (rc) return rc;
   HIPCHK(c, hipSetDevice(c->device));
-  rc = arena_reserve(c, build_scratch_bytes(n));
+  rc = arena_reserve(c, build_scratch_bytes(caps_one(n)));
   if (rc) return rc;
   return build_device_impl(c, d_values, n, rid_base, d_keys, d_offsets, d_lengths, d_words,
                            d_counts, nullptr);
@@ -401,7 +450,7 @@ int wahgpu_build(wahgpu_ctx* c, const uint32_t* values, uint64_t n, uint32_t rid
   if (rc) return rc;
   HIPCHK(c, hipSetDevice(c->device));
   uint64_t nn = n ? n : 1;
-  uint64_t bytes = build_scratch_bytes(n) + align_up(nn * 4) * 4 + align_up(2 * nn * 4) +
+  uint64_t bytes = build_scratch_bytes(caps_one(n)) + align_up(nn * 4) * 4 + align_up(2 * nn * 4) +
                    align_up(sizeof(wahgpu_counts));
   rc = arena_reserve(c, bytes);
   if (rc) return rc;
@@ -452,6 +501,251 @@ int wahgpu_build(wahgpu_ctx* c, const uint32_t* values, uint64_t n, uint32_t rid
   return WAHGPU_OK;
 }
 
+// ---- batched build -----------------------------------------------------
+
+namespace {
+
+// Validates the columns and fills the descriptor (dims + columns with tile
+// and chunk prefixes) into host memory.
+int make_desc(wahgpu_ctx* c, const wahgpu_column* cols, uint32_t ncols, uint64_t values_len,
+              std::vector<char>* out, Caps* exact) {
+  out->assign(sizeof(wah::DescHead) + (uint64_t)ncols * sizeof(wah::Column), 0);
+  auto* head = reinterpret_cast<wah::DescHead*>(out->data());
+  wah::Column* dc = wah::desc_cols(head);
+  uint64_t st = 0, tt = 0, hc = 0, end = 0;
+  for (uint32_t i = 0; i < ncols; ++i) {
+    const wahgpu_column& col = cols[i];
+    int rc = check_n(c, col.n, col.rid_base);
+    if (rc) return rc;
+    if (col.offset < end || col.offset > values_len || col.n > values_len - col.offset)
+      return fail(c, WAHGPU_EINVAL, "column " + std::to_string(i) +
+                                        " overlaps the previous one or exceeds values_len");
+    end = col.offset + col.n;
+    dc[i] = wah::Column{col.offset, col.n, col.rid_base, (uint32_t)st, (uint32_t)tt, (uint32_t)hc};
+    st += wah::sort_tiles_of(col.n);
+    tt += wah::tail_tiles_of(col.n);
+    hc += wah::hist_chunks_of(col.n);
+  }
+  if (st > 0xFFFFFFF0ull || hc > 0xFFFFFFF0ull)
+    return fail(c, WAHGPU_ERANGE, "batch too large for 32-bit tile numbering");
+  head->dims = wah::Dims{ncols, (uint32_t)st, (uint32_t)tt, (uint32_t)hc};
+  *exact = Caps{values_len, ncols, (uint32_t)st, (uint32_t)tt, (uint32_t)hc};
+  return WAHGPU_OK;
+}
+
+// Copies the descriptor to the device on the context's stream through pinned
+// staging (waits for the previous copy out of the staging buffer first).
+int upload_desc(wahgpu_ctx* c, const std::vector<char>& d, wah::DescHead* dst) {
+  if (!c->desc_copied) HIPCHK(c, hipEventCreateWithFlags(&c->desc_copied, hipEventDisableTiming));
+  HIPCHK(c, hipEventSynchronize(c->desc_copied));
+  if (d.size() > c->desc_stage_cap) {
+    if (c->desc_stage) (void)hipHostFree(c->desc_stage);
+    c->desc_stage = nullptr;
+    c->desc_stage_cap = 0;
+    uint64_t cap = align_up(d.size()) * 2;
+    HIPCHK(c, hipHostMalloc(reinterpret_cast<void**>(&c->desc_stage), cap, hipHostMallocDefault));
+    c->desc_stage_cap = cap;
+  }
+  memcpy(c->desc_stage, d.data(), d.size());
+  HIPCHK(c, hipMemcpyAsync(dst, c->desc_stage, d.size(), hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, hipEventRecord(c->desc_copied, c->stream));
+  return WAHGPU_OK;
+}
+
+// Caps of the reserved layout for graph replay.
+Caps reserved_caps(const wahgpu_ctx* c) {
+  const uint64_t v = c->res_values;
+  return Caps{v, c->res_cols, (uint32_t)(v / wah::kSortTile + c->res_cols),
+              (uint32_t)(v / wah::kTailTile + c->res_cols),
+              (uint32_t)(v / wah::kHistChunk + c->res_cols)};
+}
+
+int batch_device_impl(wahgpu_ctx* c, const uint32_t* d_values, uint64_t values_len,
+                      const wahgpu_column* cols, uint32_t ncols, uint32_t* d_keys,
+                      uint32_t* d_offsets, uint32_t* d_lengths, uint32_t* d_words,
+                      wahgpu_counts* d_counts, uint64_t extra_bytes) {
+  std::vector<char> desc;
+  Caps exact;
+  int rc = make_desc(c, cols, ncols, values_len, &desc, &exact);
+  if (rc) return rc;
+  const bool graph = c->res_cols && reserved_caps(c).covers(exact);
+  const Caps k = graph ? reserved_caps(c) : exact;
+  // extra_bytes: room the caller carves after the scratch (host batches)
+  rc = arena_reserve(c, build_scratch_bytes(k) + extra_bytes);
+  if (rc) return rc;
+  BuildArgs a{};
+  carve_build(c, &a, k);
+  a.inline_desc = false;
+  a.values = d_values;
+  a.out_keys = d_keys;
+  a.out_offsets = d_offsets;
+  a.out_lengths = d_lengths;
+  a.out_words = d_words;
+  a.counts = d_counts;
+  rc = upload_desc(c, desc, a.desc);
+  if (rc) return rc;
+  if (graph) {
+    std::vector<uintptr_t> key = {(uintptr_t)c->arena, (uintptr_t)c->stream,
+                                  (uintptr_t)d_values, (uintptr_t)d_keys,
+                                  (uintptr_t)d_offsets, (uintptr_t)d_lengths,
+                                  (uintptr_t)d_words, (uintptr_t)d_counts,
+                                  (uintptr_t)k.n, (uintptr_t)k.cols};
+    if (!c->gexec || key != c->gkey) {
+      if (c->gexec) {
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+        (void)hipGraphExecDestroy(c->gexec);
+        c->gexec = nullptr;
+      }
+      HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+      wah::launch_sort(a, c->stream);
+      wah::launch_tail(a, c->stream);
+      hipGraph_t g = nullptr;
+      hipError_t e = hipStreamEndCapture(c->stream, &g);
+      if (e != hipSuccess) return fail(c, WAHGPU_EHIP, std::string("capture: ") + hipGetErrorString(e));
+      e = hipGraphInstantiate(&c->gexec, g, nullptr, nullptr, 0);
+      (void)hipGraphDestroy(g);
+      if (e != hipSuccess) {
+        c->gexec = nullptr;
+        return fail(c, WAHGPU_EHIP, std::string("graph instantiate: ") + hipGetErrorString(e));
+      }
+      c->gkey = key;
+      ++c->captures;
+    }
+    HIPCHK(c, hipGraphLaunch(c->gexec, c->stream));
+    ++c->replays;
+    return WAHGPU_OK;
+  }
+  EventHook hook;
+  EventHook* hp = nullptr;
+  if (c->timing && c->tcalls < c->tmax) {
+    hook.ev = &c->tev[(size_t)c->tcalls * (WAHGPU_TIMING_GROUPS + 1)];
+    hp = &hook;
+    ++c->tcalls;
+    hook.mark(c->stream);
+  }
+  wah::launch_sort(a, c->stream, hp);
+  LAUNCHCHK(c);
+  wah::launch_tail(a, c->stream, hp);
+  LAUNCHCHK(c);
+  if (hp) hook.mark(c->stream);
+  return WAHGPU_OK;
+}
+
+}  // namespace
+
+int wahgpu_build_batch_device(wahgpu_ctx* c, const uint32_t* d_values, uint64_t values_len,
+                              const wahgpu_column* cols, uint32_t ncols, uint32_t* d_keys,
+                              uint32_t* d_offsets, uint32_t* d_lengths, uint32_t* d_words,
+                              wahgpu_counts* d_counts) {
+  if (!c) return WAHGPU_EINVAL;
+  if (ncols == 0) return WAHGPU_OK;
+  if (!cols || !d_counts) return fail(c, WAHGPU_EINVAL, "null columns or counts");
+  if (values_len && (!d_values || !d_keys || !d_offsets || !d_lengths || !d_words))
+    return fail(c, WAHGPU_EINVAL, "null buffer");
+  HIPCHK(c, hipSetDevice(c->device));
+  return batch_device_impl(c, d_values, values_len, cols, ncols, d_keys, d_offsets, d_lengths,
+                           d_words, d_counts, 0);
+}
+
+int wahgpu_build_batch(wahgpu_ctx* c, const uint32_t* const* values, const uint64_t* n,
+                       const uint32_t* rid_base, uint32_t ncols, wahgpu_result* results) {
+  if (!c) return WAHGPU_EINVAL;
+  if (ncols == 0) return WAHGPU_OK;
+  if (!values || !n || !results) return fail(c, WAHGPU_EINVAL, "null argument");
+  HIPCHK(c, hipSetDevice(c->device));
+  // columns at 64-value aligned offsets (16-byte vector loads per column)
+  std::vector<wahgpu_column> cols(ncols);
+  uint64_t total = 0;
+  for (uint32_t i = 0; i < ncols; ++i) {
+    if (n[i] && !values[i]) return fail(c, WAHGPU_EINVAL, "null values of a column");
+    cols[i] = wahgpu_column{total, n[i], rid_base ? rid_base[i] : 0u, 0u};
+    total += (n[i] + 63) / 64 * 64;
+  }
+  const uint64_t tn = total ? total : 1;
+  const uint64_t io = align_up(tn * 4) * 4 + align_up(2 * tn * 4) +
+                      align_up((uint64_t)ncols * sizeof(wahgpu_counts));
+  // The scratch is carved first by batch_device_impl; the io buffers follow
+  // it.  Reserve both, then carve io at the end of the scratch.
+  std::vector<char> desc;
+  Caps exact;
+  int rc = make_desc(c, cols.data(), ncols, total, &desc, &exact);
+  if (rc) return rc;
+  const bool graph = c->res_cols && reserved_caps(c).covers(exact);
+  const uint64_t scratch = build_scratch_bytes(graph ? reserved_caps(c) : exact);
+  rc = arena_reserve(c, scratch + io);
+  if (rc) return rc;
+  c->bump = scratch;
+  uint32_t* d_values = take<uint32_t>(c, tn);
+  uint32_t* d_keys = take<uint32_t>(c, tn);
+  uint32_t* d_offsets = take<uint32_t>(c, tn);
+  uint32_t* d_lengths = take<uint32_t>(c, tn);
+  uint32_t* d_words = take<uint32_t>(c, 2 * tn);
+  wahgpu_counts* d_counts = take<wahgpu_counts>(c, ncols);
+  for (uint32_t i = 0; i < ncols; ++i)
+    if (n[i])
+      HIPCHK(c, hipMemcpyAsync(d_values + cols[i].offset, values[i], n[i] * 4,
+                               hipMemcpyHostToDevice, c->stream));
+  rc = batch_device_impl(c, d_values, total, cols.data(), ncols, d_keys, d_offsets, d_lengths,
+                         d_words, d_counts, io);
+  if (rc) return rc;
+  std::vector<wahgpu_counts> hc(ncols);
+  HIPCHK(c, hipMemcpyAsync(hc.data(), d_counts, ncols * sizeof(wahgpu_counts),
+                           hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  for (uint32_t i = 0; i < ncols; ++i) {
+    const wahgpu_counts& k = hc[i];
+    wahgpu_result& r = results[i];
+    r.counts = k;
+    if (k.fault)
+      return fail(c, WAHGPU_EHIP, "kernel reported fault " + std::to_string(k.fault) +
+                                      " (look-back wait gave up or index out of range)");
+    if (k.keycnt > r.keys_cap || k.length > r.words_cap)
+      return fail(c, WAHGPU_EINVAL, "output capacity too small for column " + std::to_string(i));
+    if ((k.length && !r.words) || (k.keycnt && (!r.keys || !r.offsets || !r.lengths)))
+      return fail(c, WAHGPU_EINVAL, "null output of column " + std::to_string(i));
+    const uint64_t o = cols[i].offset;
+    if (k.length)
+      HIPCHK(c, hipMemcpyAsync(r.words, d_words + 2 * o, k.length * 4, hipMemcpyDeviceToHost,
+                               c->stream));
+    if (k.keycnt) {
+      HIPCHK(c, hipMemcpyAsync(r.keys, d_keys + o, k.keycnt * 4, hipMemcpyDeviceToHost, c->stream));
+      HIPCHK(c, hipMemcpyAsync(r.offsets, d_offsets + o, k.keycnt * 4, hipMemcpyDeviceToHost,
+                               c->stream));
+      HIPCHK(c, hipMemcpyAsync(r.lengths, d_lengths + o, k.keycnt * 4, hipMemcpyDeviceToHost,
+                               c->stream));
+    }
+  }
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return WAHGPU_OK;
+}
+
+int wahgpu_batch_reserve(wahgpu_ctx* c, uint64_t max_values, uint32_t max_cols) {
+  if (!c) return WAHGPU_EINVAL;
+  if ((max_values == 0) != (max_cols == 0))
+    return fail(c, WAHGPU_EINVAL, "max_values and max_cols are both zero or both nonzero");
+  HIPCHK(c, hipSetDevice(c->device));
+  if (c->gexec) {
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    (void)hipGraphExecDestroy(c->gexec);
+    c->gexec = nullptr;
+    c->gkey.clear();
+  }
+  c->res_values = max_values;
+  c->res_cols = max_cols;
+  if (max_cols) {
+    int rc = arena_reserve(c, build_scratch_bytes(reserved_caps(c)));
+    if (rc) return rc;
+  }
+  return WAHGPU_OK;
+}
+
+int wahgpu_graph_stats(const wahgpu_ctx* c, uint64_t* captures, uint64_t* replays) {
+  if (!c) return WAHGPU_EINVAL;
+  if (captures) *captures = c->captures;
+  if (replays) *replays = c->replays;
+  return WAHGPU_OK;
+}
+
 // ---- staged build (one kernel group per reference stage) ------------------
 
 int wahgpu_build_staged(wahgpu_ctx* c, const uint32_t* values, uint64_t n, uint32_t* keys,
@@ -464,13 +758,14 @@ int wahgpu_build_staged(wahgpu_ctx* c, const uint32_t* values, uint64_t n, uint3
   if (rc) return rc;
   HIPCHK(c, hipSetDevice(c->device));
   uint64_t nn = n ? n : 1;
-  uint64_t bytes = build_scratch_bytes(n) + align_up(nn * 4) * 12 + align_up(2 * nn * 4) +
+  uint64_t bytes = build_scratch_bytes(caps_one(n)) + align_up(nn * 4) * 12 + align_up(2 * nn * 4) +
                    align_up(sizeof(wahgpu_counts)) + 4 * scan_scratch_bytes(2 * nn);
   rc = arena_reserve(c, bytes);
   if (rc) return rc;
   auto t0 = std::chrono::steady_clock::now();
   BuildArgs a{};
-  carve_build(c, &a, n);
+  carve_build(c, &a, caps_one(n));
+  set_one(&a, n, 0);
   uint32_t* d_values = take<uint32_t>(c, nn);
   uint32_t* s_keys = take<uint32_t>(c, nn);
   uint32_t* s_rids = take<uint32_t>(c, nn);
@@ -492,8 +787,6 @@ int wahgpu_build_staged(wahgpu_ctx* c, const uint32_t* values, uint64_t n, uint3
   if (n) HIPCHK(c, hipMemcpyAsync(d_values, values, n * 4, hipMemcpyHostToDevice, c->stream));
   m.mark(c->stream);  // 1 transfer to
   a.values = d_values;
-  a.n = n;
-  a.rid_base = 0;
   a.counts = d_counts;
   a.sorted_keys = s_keys;
   a.sorted_rids = s_rids;
@@ -573,13 +866,12 @@ int wahgpu_sort_rids_by_value(wahgpu_ctx* c, const uint32_t* d_values, uint64_t 
   int rc = check_n(c, n, rid_base);
   if (rc) return rc;
   HIPCHK(c, hipSetDevice(c->device));
-  rc = arena_reserve(c, build_scratch_bytes(n) + align_up(sizeof(wahgpu_counts)));
+  rc = arena_reserve(c, build_scratch_bytes(caps_one(n)) + align_up(sizeof(wahgpu_counts)));
   if (rc) return rc;
   BuildArgs a{};
-  carve_build(c, &a, n);
+  carve_build(c, &a, caps_one(n));
+  set_one(&a, n, rid_base);
   a.values = d_values;
-  a.n = n;
-  a.rid_base = rid_base;
   a.counts = take<wahgpu_counts>(c, 1);
   a.sorted_keys = d_keys_out;
   a.sorted_rids = d_rids_out;

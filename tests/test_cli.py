@@ -72,7 +72,7 @@ def test_phases_requires_a_file():
 
 
 def test_help_flags():
-    for prog in ("phases", "gpu", "generate"):
+    for prog in ("phases", "gpu", "generate", "scan", "sc", "segmented_scan"):
         out = run(os.path.join(BIN, prog), "--help").stdout
         assert "--" in out
 
@@ -139,3 +139,27 @@ def test_gpu_program_builtin_sample():
     for line in got:
         k, bits = line.split("\t")
         assert bits == refd.get(k, "")
+
+
+# ---- primitive checks (src/scan.cpp, src/sc.cpp, src/segmented_scan.cpp) -------
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("size", [1, 5000, 1048577 + 33])
+def test_primitive_programs_self_check(size):
+    out = run(os.path.join(BIN, "scan"), "--seed", "3", "--size", str(size)).stdout.splitlines()
+    assert out == [f"Using {size} values.", "Success!"]
+    out = run(os.path.join(BIN, "sc"), "--seed", "4", "--size", str(size), "-F", "0.3").stdout
+    assert out.splitlines() == [f"Compacting {size} values.", "SUCCESS"]
+    out = run(os.path.join(BIN, "segmented_scan"), "--seed", "5", "--size", str(size)).stdout
+    assert out.splitlines() == [f"Using {size} values.", "Success"]
+
+
+@pytest.mark.gpu
+def test_primitive_programs_on_a_file(tmp_path):
+    values = np.random.default_rng(9).integers(0, 1 << 32, 70001, dtype=np.uint64)
+    path = value_file(tmp_path, values.astype(np.uint32))
+    assert run(os.path.join(BIN, "scan"), "-f", path).stdout.splitlines() == ["Success!"]
+    assert run(os.path.join(BIN, "sc"), "-f", path, "--seed", "1").stdout.splitlines() == ["SUCCESS"]
+    out = run(os.path.join(BIN, "segmented_scan"), "-f", path, "--seed", "2", "-p").stdout
+    lines = out.splitlines()
+    assert lines[0] == "Success" and len(lines) == 1 + values.size

@@ -1,0 +1,148 @@
+"""Batched builds (many columns in one launch set) against the oracle and
+against single-column builds.
+
+The reference indexes one slice per worker request (src/gpu.cpp:194-227,
+kernel_wah_index); a batch must give every column exactly the index the
+oracle (src/cpu.cpp:313-328) gives that column alone, whatever its
+neighbours: tiles never straddle columns and look-back chains stay inside a
+column.  Graph replays are checked call by call against eager builds, with
+eager work between the replays.
+"""
+import numpy as np
+import pytest
+
+import indexing_amd as ia
+from oracle import oracle
+from tests.test_gpu_parity import assert_same, dev, u32
+
+pytestmark = pytest.mark.gpu
+
+
+def mixed_columns(seed, count, max_log=17):
+    """Columns of log-uniform length and cardinality, half uniform half Zipf,
+    plus the edge cases: empty, one value, constant, full 32-bit keys."""
+    rng = np.random.default_rng(seed)
+    cols = [np.zeros(0, np.uint32), np.array([9], np.uint32), np.full(5000, 3, np.uint32),
+            rng.integers(0, 1 << 32, 3000, dtype=np.uint64).astype(np.uint32)]
+    for i in range(count):
+        n = int(2 ** rng.uniform(0, max_log))
+        card = int(2 ** rng.uniform(1, 16))
+        if i % 2:
+            cols.append(rng.integers(0, card, n).astype(np.uint32))
+        else:
+            cols.append(ia.generate(n, "zipf", seed=seed * 1000 + i, cardinality=card, zipf_s=1.0))
+    order = rng.permutation(len(cols))
+    return [cols[i] for i in order]
+
+
+def per_column(d_out, counts, layout):
+    keys, offs, lens, words, _ = (u32(t) for t in d_out)
+    res = []
+    for c, (off, n) in zip(counts, layout):
+        K, L = int(c.keycnt), int(c.length)
+        res.append(ia.Index(keys[off:off + K], offs[off:off + K], lens[off:off + K],
+                            words[2 * off:2 * off + L], int(c.k), L, K, int(c.bound),
+                            int(c.passes)))
+    return res
+
+
+def pack(cols, gap=0, align=1):
+    """Concatenate columns (optionally with gaps / odd alignment)."""
+    layout, parts, pos = [], [], 0
+    for c in cols:
+        pos = (pos + align - 1) // align * align
+        layout.append((pos, c.size))
+        parts.append((pos, c))
+        pos += c.size + gap
+    buf = np.full(max(pos, 1), 0xDEADBEEF, np.uint32)
+    for p, c in parts:
+        buf[p:p + c.size] = c
+    return buf, layout
+
+
+@pytest.mark.parametrize("seed,gap,align", [(1, 0, 1), (2, 3, 1), (3, 0, 64), (4, 7, 4)])
+def test_batch_device_matches_oracle(ctx, seed, gap, align):
+    cols = mixed_columns(seed, 40, max_log=16)
+    buf, layout = pack(cols, gap, align)
+    rid_bases = [(i * 1000003) % (1 << 20) for i in range(len(cols))]
+    print(f"[batch] seed={seed} cols={len(cols)} total={buf.size}", flush=True)
+    spec = [(o, n, r) for (o, n), r in zip(layout, rid_bases)]
+    out = ctx.build_batch_device(dev(buf), spec)
+    counts = ctx.read_counts_batch(out[4], len(cols))
+    got = per_column(out, counts, layout)
+    for i, (c, g, r) in enumerate(zip(cols, got, rid_bases)):
+        assert counts[i].fault == 0
+        assert_same(g, oracle.build(c, rid_base=r), f"column {i} n={c.size}")
+
+
+def test_batch_equals_single_builds_large(ctx):
+    """Columns large enough for multi-tile look-back chains and 3-4 passes."""
+    rng = np.random.default_rng(11)
+    cols = [rng.integers(0, 65536, 300000).astype(np.uint32),
+            rng.integers(0, 1 << 32, 120001, dtype=np.uint64).astype(np.uint32),
+            rng.integers(0, 3, 250000).astype(np.uint32),
+            ia.generate(400000, "zipf-runs", seed=4, cardinality=1 << 20, zipf_s=1.2,
+                        run_mean=500),
+            rng.integers(0, 1 << 24, 77777).astype(np.uint32)]
+    buf, layout = pack(cols)
+    print(f"[batch-large] total={buf.size}", flush=True)
+    out = ctx.build_batch_device(dev(buf), layout)
+    counts = ctx.read_counts_batch(out[4], len(cols))
+    got = per_column(out, counts, layout)
+    for i, c in enumerate(cols):
+        single = ctx.build(c)
+        assert_same(got[i], oracle.build(c), f"column {i}")
+        assert np.array_equal(got[i].words, single.words)
+        assert got[i].passes == single.passes
+
+
+def test_batch_host_api(ctx):
+    cols = mixed_columns(5, 25, max_log=15)
+    rid_bases = [7 * i for i in range(len(cols))]
+    print(f"[batch-host] cols={len(cols)}", flush=True)
+    got = ctx.build_batch(cols, rid_bases)
+    for i, (c, g) in enumerate(zip(cols, got)):
+        assert_same(g, oracle.build(c, rid_base=rid_bases[i]), f"column {i}")
+
+
+def test_batch_rejects_overlap_and_bad_ranges(ctx):
+    buf = dev(np.arange(100, dtype=np.uint32))
+    with pytest.raises(ia.WahError):
+        ctx.build_batch_device(buf, [(0, 60), (50, 10)])
+    with pytest.raises(ia.WahError):
+        ctx.build_batch_device(buf, [(90, 20)])
+    with pytest.raises(ia.WahError):
+        ctx.build_batch_device(buf, [(0, 10, 0xFFFFFFFF)])
+
+
+def test_graph_replays_match_eager(ctx):
+    """One captured graph replayed over different batches: each replay equals
+    the eager batched build and the oracle."""
+    import torch
+    g = ia.Context(0)
+    try:
+        g.batch_reserve(1 << 20, 64)
+        cap = 1 << 20
+        outs = (g._empty(cap), g._empty(cap), g._empty(cap), g._empty(2 * cap),
+                torch.zeros(64 * 10, dtype=torch.int32, device="cuda:0"))
+        d_vals = torch.zeros(cap, dtype=torch.int32, device="cuda:0")
+        for rep in range(4):
+            cols = mixed_columns(100 + rep, 20 + 5 * rep, max_log=15)
+            buf, layout = pack(cols, gap=rep)
+            assert buf.size <= cap and len(cols) <= 64
+            d_vals[:buf.size].copy_(dev(buf))
+            print(f"[graph] replay {rep} cols={len(cols)} total={buf.size}", flush=True)
+            g.build_batch_device(d_vals[:cap], layout, out=outs)
+            counts = g.read_counts_batch(outs[4], len(cols))
+            got = per_column(outs, counts, layout)
+            eager = ctx.build_batch_device(d_vals[:buf.size].clone(), layout)
+            ecounts = ctx.read_counts_batch(eager[4], len(cols))
+            egot = per_column(eager, ecounts, layout)
+            for i, c in enumerate(cols):
+                assert_same(got[i], oracle.build(c), f"replay {rep} column {i}")
+                assert np.array_equal(got[i].words, egot[i].words)
+            torch.cuda.synchronize()
+        captures, replays = g.graph_stats()
+        assert captures == 1 and replays == 4
+    finally:
+        g.close()

@@ -82,7 +82,8 @@ __host__ __device__ inline Column* desc_cols(DescHead* d) {
 struct Header {
   uint32_t tiles[kCounters][kLaneWords];  // tile-counter lanes per kernel
   uint32_t fault;                         // nonzero after a failed wait / bad index
-  uint32_t pad[63];
+  uint32_t max_active;                    // most radix passes any column needs
+  uint32_t pad[62];
 };
 
 // Per-column state.  hist and maxval are zeroed by k_begin; the rest is

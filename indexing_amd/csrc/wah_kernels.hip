@@ -161,7 +161,7 @@ __global__ __launch_bounds__(kBlock) void k_histogram(
 // its own so the histogram blocks need no device-scope fences (on gfx950
 // each one writes back and invalidates the L2).
 __global__ __launch_bounds__(kBlock) void k_plan(const DescHead* __restrict__ desc,
-                                                 ColState* __restrict__ cs,
+                                                 ColState* __restrict__ cs, Header* __restrict__ hdr,
                                                  wahgpu_counts_dev* __restrict__ counts) {
   __shared__ uint64_t s_scan[kWaves];
   __shared__ int s_trivial[4];
@@ -187,6 +187,7 @@ __global__ __launch_bounds__(kBlock) void k_plan(const DescHead* __restrict__ de
       for (uint32_t q = a; q < 4; ++q) col->shift[q] = 0;
       col->num_active = a;
       col->final_buf = (a == 0) ? kBufValues : ((a - 1) % 2 == 0 ? kBufA : kBufB);
+      if (a) atomicMax(&hdr->max_active, a);
       wahgpu_counts_dev z;
       z.k = 0; z.length = 0; z.keycnt = 0;
       z.bound = col->maxval; z.passes = a; z.fault = 0; z.reserved = 0;
@@ -233,6 +234,7 @@ __global__ __launch_bounds__(THREADS, WAH_SORT_WAVES_PER_EU) void k_onesweep(
   const Dims dims = desc->dims;
   const Column* cols = desc_cols(desc);
   const int t = threadIdx.x, w = t / kWave, l = t % kWave;
+  if (hdr->max_active <= a) return;  // no column has an a-th pass
 
   uint32_t iter = 0;
   for (;;) {
@@ -913,7 +915,7 @@ void launch_sort(const BuildArgs& a, hipStream_t s, LaunchHook* hook) {
   hipLaunchKernelGGL(k_histogram, dim3(cap_grid((uint32_t)hb, kHistMaxBlocks)), dim3(kBlock), 0,
                      s, a.values, a.desc, a.cs, a.sort_status, a.tail_status);
   hipLaunchKernelGGL(k_plan, dim3(cap_grid(a.cols_cap, 1024)), dim3(kBlock), 0, s, a.desc, a.cs,
-                     a.counts);
+                     a.hdr, a.counts);
   if (hook) hook->mark(s);
   const uint32_t grid =
       cap_grid(a.sort_tiles_cap, WAH_STATIC_TILES ? a.sort_tiles_cap

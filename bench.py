@@ -15,6 +15,16 @@ Beside it:
   cpu_baseline  the oracle (a C restatement of the reference's serial `cpu`
                 program, one thread) on a 2^26-value sample of the same
                 distribution, timed on this host
+  batched       the mixed 1024-column workload (lengths log-uniform in
+                [2^12, 2^24], cardinality log-uniform in [2, 2^16], half
+                uniform half Zipf(1.0), seed 7), resident in HBM: one batched
+                build (graph replay) against a loop of per-column
+                wahgpu_build_device calls over the same columns
+  serve         requests drawn from the same distributions (seed 11), host
+                memory in and out: the continuous-batching server against the
+                plain per-column drop-in (wahgpu_build per request), first as
+                a burst (saturation throughput), then as an open-loop Poisson
+                stream at half the plain path's saturation rate (p50 / p99)
 Prints one JSON line.
 """
 import argparse
@@ -45,6 +55,10 @@ def read_traffic():
     return None
 
 
+def progress(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
 def cpu_baseline(sample_n):
     import numpy as np
 
@@ -61,6 +75,120 @@ def cpu_baseline(sample_n):
                       f"{dt:.2f} s"}
 
 
+def bench_batched(ncols, seed=7, reps=3):
+    """C4: one batched build of ncols mixed columns vs per-column builds."""
+    import numpy as np
+    import torch
+
+    import indexing_amd as ia
+    from indexing_amd import workloads as W
+    specs = W.column_specs(ncols, seed)
+    progress(f"batched: generating {ncols} columns")
+    t0 = time.perf_counter()
+    buf, layout = W.make_columns(specs)
+    gen_s = time.perf_counter() - t0
+    progress(f"batched: {gen_s:.1f} s to generate, building")
+    total = int(sum(n for _, n in layout))
+    d_vals = torch.from_numpy(buf.view(np.int32)).to("cuda:0")
+    del buf
+    ctx = ia.Context(0)
+    ctx.batch_reserve(d_vals.numel(), ncols)
+    out = ctx.build_batch_device(d_vals, layout)          # captures the graph
+    torch.cuda.synchronize()
+    counts = ctx.read_counts_batch(out[4], ncols)
+    assert all(c.fault == 0 for c in counts)
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
+    best_b = 1e30
+    for _ in range(reps):
+        ev[0].record()
+        ctx.build_batch_device(d_vals, layout, out=out)
+        ev[1].record()
+        torch.cuda.synchronize()
+        best_b = min(best_b, ev[0].elapsed_time(ev[1]))
+    # plain: one fused build per column on the same device data and buffers
+    keys, offs, lens, words, _ = out
+    cnt = torch.zeros(ncols * 10, dtype=torch.int32, device="cuda:0")
+    best_p = 1e30
+    for _ in range(reps):
+        ev[0].record()
+        for i, (o, n) in enumerate(layout):
+            ctx.build_device(d_vals[o:o + n], out=(keys[o:o + n], offs[o:o + n], lens[o:o + n],
+                                                   words[2 * o:2 * (o + n)],
+                                                   cnt[10 * i:10 * i + 10]))
+        ev[1].record()
+        torch.cuda.synchronize()
+        best_p = min(best_p, ev[0].elapsed_time(ev[1]))
+    # same results both ways (counts of every column)
+    pc = [ctx.read_counts(cnt[10 * i:10 * i + 10]) for i in range(ncols)]
+    assert all((a.length, a.keycnt, a.k) == (b.length, b.keycnt, b.k) for a, b in zip(counts, pc))
+    captures, replays = ctx.graph_stats()
+    ctx.close()
+    del d_vals, out
+    torch.cuda.empty_cache()
+    return {"workload": f"{ncols} columns, lengths log-uniform [2^12, 2^24], cardinality "
+                        "log-uniform [2, 2^16], half uniform half Zipf(1.0), seed 7, resident",
+            "columns": ncols, "values": total, "batched_ms": round(best_b, 3),
+            "batched_gvalues_s": round(total / best_b / 1e6, 2),
+            "per_column_ms": round(best_p, 3),
+            "per_column_gvalues_s": round(total / best_p / 1e6, 2),
+            "speedup": round(best_p / best_b, 2), "graph_captures": captures,
+            "graph_replays": replays, "host_generation_s": round(gen_s, 1)}
+
+
+def bench_serve(nreq, seed=11, arena_gib=64):
+    """C5: continuous-batching server vs plain per-column path on host
+    requests; burst throughput, then Poisson arrivals at a fixed rate."""
+    import numpy as np
+
+    import indexing_amd as ia
+    from indexing_amd import workloads as W
+    specs = W.column_specs(nreq, seed)
+    progress(f"serve: generating {nreq} requests")
+    buf, layout = W.make_columns(specs)
+    cols = [buf[o:o + n] for o, n in layout]
+    total = int(sum(n for _, n in layout))
+
+    def run(srv, arrivals):
+        progress(f"serve: {len(arrivals)} requests")
+        srv.reset_stats()
+        t0 = time.perf_counter()
+        pend = []
+        for c, at in zip(cols, arrivals):
+            delay = t0 + at - time.perf_counter()
+            if delay > 0:
+                time.sleep(delay)
+            pend.append(srv.submit(c))
+        for p in pend:
+            srv.wait(p)
+        st = srv.stats()
+        return {"p50_ms": round(st["p50_us"] / 1e3, 3), "p99_ms": round(st["p99_us"] / 1e3, 3),
+                "gvalues_s": round(st["values"] / st["busy_s"] / 1e9, 3),
+                "batches": int(st["batches"]), "max_batch_cols": int(st["max_batch_cols"])}
+
+    burst = [0.0] * nreq
+    res = {}
+    with ia.Server(slots=1, arena_bytes=8 << 30, plain=True) as plain:
+        run(plain, burst[:8])  # warm-up (allocations)
+        res["plain_burst"] = run(plain, burst)
+    with ia.Server(slots=2, arena_bytes=arena_gib << 30, max_batch_cols=1024) as srv:
+        run(srv, burst[:8])
+        res["served_burst"] = run(srv, burst)
+        rate = res["plain_burst"]["gvalues_s"] * 1e9 * 0.5 / (total / nreq)  # requests/s
+        gaps = np.random.default_rng(seed).exponential(1.0 / rate, nreq)
+        arrivals = np.cumsum(gaps) - gaps[0]
+        res["served_poisson"] = run(srv, arrivals)
+    with ia.Server(slots=1, arena_bytes=8 << 30, plain=True) as plain:
+        run(plain, burst[:8])
+        res["plain_poisson"] = run(plain, arrivals)
+    res.update({"requests": nreq, "values": total, "rate_req_s": round(rate, 1),
+                "workload": f"{nreq} requests, lengths log-uniform [2^12, 2^24], cardinality "
+                            "log-uniform [2, 2^16], half uniform half Zipf(1.0), seed 11, "
+                            f"host memory, {arena_gib} GiB arena, 2 slots",
+                "throughput_ratio_burst": round(res["served_burst"]["gvalues_s"] /
+                                                res["plain_burst"]["gvalues_s"], 2)})
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -68,6 +196,10 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--n", type=int, default=N_VALUES, help="column length (default 2^28)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--batch-cols", type=int, default=1024,
+                    help="columns of the batched workload (0 skips it)")
+    ap.add_argument("--serve-requests", type=int, default=400,
+                    help="requests of the serving workload (0 skips it)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
@@ -146,9 +278,16 @@ def main():
         "kernel_ms": {name: round(g / calls, 4) for name, g in zip(
             ["histogram", "pass0", "pass1", "pass2", "pass3", "tail", "lengths"], group_ms)},
     }
-    if not args.no_cpu_baseline:
-        result["cpu_baseline"] = cpu_baseline(min(CPU_SAMPLE, n))
     ctx.close()
+    del d_values, out
+    torch.cuda.empty_cache()
+    if args.batch_cols:
+        result["batched"] = bench_batched(args.batch_cols)
+    if args.serve_requests:
+        result["serve"] = bench_serve(args.serve_requests)
+    if not args.no_cpu_baseline:
+        progress("cpu baseline")
+        result["cpu_baseline"] = cpu_baseline(min(CPU_SAMPLE, n))
     print(json.dumps(result))
 
 

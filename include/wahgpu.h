@@ -179,6 +179,60 @@ int wahgpu_batch_reserve(wahgpu_ctx* ctx, uint64_t max_values, uint32_t max_cols
 /* Number of graph captures and replays so far (for tests and tuning). */
 int wahgpu_graph_stats(const wahgpu_ctx* ctx, uint64_t* captures, uint64_t* replays);
 
+/* ---- serving: continuous batching of build requests ----------------------
+ * A host request loop in place of the indexer -> worker round trips
+ * (src/gpu.cpp:194-261): requests (one column each, host memory) queue as
+ * they arrive; `slots` worker threads, each with its own stream, device
+ * buffers and captured graph, take every queued request that fits their
+ * batch (admission by HBM footprint: ~45 bytes per value plus ~8.5 KiB per
+ * column, within arena_bytes / slots) and run it as one batched build, so
+ * requests arriving while others are on the GPU join the next batch of the
+ * first slot to free up, and the copies and kernels of different slots
+ * overlap on the device.  With plain = 1 every slot instead serves one
+ * request at a time through wahgpu_build (the per-column drop-in), for
+ * comparison under the same arrivals. */
+typedef struct wahgpu_server wahgpu_server;
+typedef struct wahgpu_server_config {
+  int device;
+  uint32_t slots;            /* worker slots (batches in flight), >= 1 */
+  uint64_t arena_bytes;      /* device memory for all slots */
+  uint32_t max_batch_cols;   /* columns per batch */
+  uint64_t max_batch_values; /* values per batch; 0 = what arena_bytes allows */
+  uint32_t plain;            /* 1: one wahgpu_build per request */
+  uint32_t reserved;
+} wahgpu_server_config;
+
+typedef struct wahgpu_server_stats {
+  uint64_t requests;      /* completed */
+  uint64_t batches;
+  uint64_t values;        /* values of completed requests */
+  uint64_t max_batch_cols;
+  uint64_t slot_values;   /* admission limit per batch, in values */
+  double p50_us, p99_us, mean_us, max_us;  /* submit -> result in host buffers */
+  double busy_s;          /* first submit -> last completion */
+} wahgpu_server_stats;
+
+int wahgpu_server_create(wahgpu_server** out, const wahgpu_server_config* cfg);
+/* Stops the slots after the queued requests are served. */
+void wahgpu_server_destroy(wahgpu_server* s);
+/* Queues one column.  values must stay valid and the result buffers
+ * (capacities as for wahgpu_build) untouched until the request completes;
+ * result->counts is filled then.  Fails with WAHGPU_ERANGE for a column that
+ * exceeds a slot.  *id names the request. */
+int wahgpu_server_submit(wahgpu_server* s, const uint32_t* values, uint64_t n,
+                         uint32_t rid_base, wahgpu_result* result, uint64_t* id);
+/* WAHGPU_OK when request id has completed, WAHGPU_EBUSY while pending,
+ * the request's error code if it failed. */
+int wahgpu_server_poll(wahgpu_server* s, uint64_t id);
+/* Blocks until request id completes; returns as poll.  Completed requests
+ * are forgotten once polled or waited for with a final status. */
+int wahgpu_server_wait(wahgpu_server* s, uint64_t id);
+/* Blocks until every submitted request has completed. */
+int wahgpu_server_drain(wahgpu_server* s);
+int wahgpu_server_get_stats(wahgpu_server* s, wahgpu_server_stats* out);
+/* Clears the latency record and counters (between measurement phases). */
+int wahgpu_server_reset_stats(wahgpu_server* s);
+
 /* ---- stages (device pointers; each synchronizes when it returns a count) */
 /* include/sort_rids_by_value.cl:24-30 + include/radix.cl:65-233: stable sort
  * of (value, rid = rid_base + i) by value. */

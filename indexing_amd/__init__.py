@@ -21,7 +21,7 @@ from . import _native as N
 from ._native import WahError, MAX_VALUES
 
 __all__ = [
-    "Context", "Index", "WahError", "MAX_VALUES", "Merger", "read_values_text",
+    "Context", "Server", "Index", "WahError", "MAX_VALUES", "Merger", "read_values_text",
     "read_values_binary", "decode_bitmap", "generate", "device_count",
 ]
 
@@ -378,14 +378,19 @@ def decode_bitmap(words):
 
 
 def generate(n, dist="uniform", seed=0, lo=0, hi=65535, cardinality=1 << 20, zipf_s=1.2,
-             run_mean=4096.0, start=0):
-    """Seeded synthetic column (the `generate` program's distributions)."""
+             run_mean=4096.0, start=0, out=None):
+    """Seeded synthetic column (the `generate` program's distributions).
+    With out (a contiguous uint32 array of at least n entries) the values are
+    written there."""
     p = N.GenParams()
     p.dist = {"uniform": N.DIST_UNIFORM, "zipf": N.DIST_ZIPF,
               "zipf-runs": N.DIST_ZIPF_RUNS}[dist]
     p.from_, p.to, p.cardinality = lo, hi, cardinality
     p.zipf_s, p.run_mean, p.seed = zipf_s, run_mean, seed
-    out = np.empty(max(int(n), 1), np.uint32)
+    if out is None:
+        out = np.empty(max(int(n), 1), np.uint32)
+    elif out.dtype != np.uint32 or not out.flags.c_contiguous or out.size < n:
+        raise ValueError("out must be a contiguous uint32 array of at least n entries")
     rc = N.lib().wahgpu_generate_range(ctypes.byref(p), out.ctypes.data_as(ctypes.c_void_p),
                                        int(start), int(n))
     if rc != N.OK:
@@ -426,3 +431,98 @@ class Merger:
         self._lib.wahgpu_merger_export(self._h, *(a.ctypes.data_as(ctypes.c_void_p)
                                                   for a in (keys, offs, lens, words)))
         return Index(keys[:K], offs[:K], lens[:K], words[:L], 0, L, K)
+
+
+class _Pending:
+    """A submitted request: keeps the values and result buffers alive."""
+    __slots__ = ("id", "values", "bufs", "res")
+
+
+class Server:
+    """Continuous-batching build server (wahgpu_server_*): requests queue as
+    they are submitted; worker slots build every queued request that fits
+    their HBM budget as one batch.  plain=True serves one request at a time
+    through the per-column drop-in build instead."""
+
+    def __init__(self, device=0, slots=2, arena_bytes=8 << 30, max_batch_cols=1024,
+                 max_batch_values=0, plain=False):
+        self._lib = N.lib()
+        cfg = N.ServerConfig(int(device), int(slots), int(arena_bytes), int(max_batch_cols),
+                             int(max_batch_values), 1 if plain else 0, 0)
+        h = ctypes.c_void_p()
+        rc = self._lib.wahgpu_server_create(ctypes.byref(h), ctypes.byref(cfg))
+        if rc != N.OK:
+            raise WahError(rc, "wahgpu_server_create failed")
+        self._h = h
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self._lib.wahgpu_server_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def submit(self, values, rid_base=0):
+        p = _Pending()
+        p.values = _as_u32(values)
+        cap = max(p.values.size, 1)
+        p.bufs = (np.empty(cap, np.uint32), np.empty(cap, np.uint32), np.empty(cap, np.uint32),
+                  np.empty(2 * cap, np.uint32))
+        r = N.Result()
+        r.keys, r.offsets, r.lengths = (x.ctypes.data for x in p.bufs[:3])
+        r.keys_cap = cap
+        r.words = p.bufs[3].ctypes.data
+        r.words_cap = 2 * cap
+        p.res = r
+        rid = ctypes.c_uint64(0)
+        rc = self._lib.wahgpu_server_submit(self._h, p.values.ctypes.data_as(ctypes.c_void_p),
+                                            p.values.size, int(rid_base), ctypes.byref(r),
+                                            ctypes.byref(rid))
+        if rc != N.OK:
+            raise WahError(rc, "wahgpu_server_submit failed")
+        p.id = rid.value
+        return p
+
+    def _index(self, p):
+        c = p.res.counts
+        K, L = int(c.keycnt), int(c.length)
+        k, o, ln, w = p.bufs
+        return Index(k[:K], o[:K], ln[:K], w[:L], int(c.k), L, K, int(c.bound), int(c.passes))
+
+    def poll(self, p):
+        """-> Index when done, None while pending (raises on failure)."""
+        rc = self._lib.wahgpu_server_poll(self._h, p.id)
+        if rc == N.EBUSY:
+            return None
+        if rc != N.OK:
+            raise WahError(rc, f"request {p.id} failed")
+        return self._index(p)
+
+    def wait(self, p):
+        rc = self._lib.wahgpu_server_wait(self._h, p.id)
+        if rc != N.OK:
+            raise WahError(rc, f"request {p.id} failed")
+        return self._index(p)
+
+    def drain(self):
+        rc = self._lib.wahgpu_server_drain(self._h)
+        if rc != N.OK:
+            raise WahError(rc, "drain failed")
+
+    def stats(self):
+        st = N.ServerStats()
+        self._lib.wahgpu_server_get_stats(self._h, ctypes.byref(st))
+        return {name: getattr(st, name) for name, _ in N.ServerStats._fields_}
+
+    def reset_stats(self):
+        self._lib.wahgpu_server_reset_stats(self._h)

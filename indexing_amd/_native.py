@@ -53,6 +53,21 @@ class Result(ctypes.Structure):
                 ("words", vp), ("words_cap", ctypes.c_uint64), ("counts", Counts)]
 
 
+class ServerConfig(ctypes.Structure):
+    _fields_ = [("device", ctypes.c_int), ("slots", ctypes.c_uint32),
+                ("arena_bytes", ctypes.c_uint64), ("max_batch_cols", ctypes.c_uint32),
+                ("max_batch_values", ctypes.c_uint64), ("plain", ctypes.c_uint32),
+                ("reserved", ctypes.c_uint32)]
+
+
+class ServerStats(ctypes.Structure):
+    _fields_ = [("requests", ctypes.c_uint64), ("batches", ctypes.c_uint64),
+                ("values", ctypes.c_uint64), ("max_batch_cols", ctypes.c_uint64),
+                ("slot_values", ctypes.c_uint64), ("p50_us", ctypes.c_double),
+                ("p99_us", ctypes.c_double), ("mean_us", ctypes.c_double),
+                ("max_us", ctypes.c_double), ("busy_s", ctypes.c_double)]
+
+
 # name -> (restype, argtypes); every function the header declares.
 SIGNATURES = {
     "wahgpu_device_count": (ctypes.c_int, [ctypes.POINTER(ctypes.c_int)]),
@@ -85,6 +100,15 @@ SIGNATURES = {
                                           ctypes.c_uint32, ctypes.POINTER(Result)]),
     "wahgpu_batch_reserve": (ctypes.c_int, [vp, ctypes.c_uint64, ctypes.c_uint32]),
     "wahgpu_graph_stats": (ctypes.c_int, [vp, c_u64p, c_u64p]),
+    "wahgpu_server_create": (ctypes.c_int, [ctypes.POINTER(vp), ctypes.POINTER(ServerConfig)]),
+    "wahgpu_server_destroy": (None, [vp]),
+    "wahgpu_server_submit": (ctypes.c_int, [vp, vp, ctypes.c_uint64, ctypes.c_uint32,
+                                            ctypes.POINTER(Result), c_u64p]),
+    "wahgpu_server_poll": (ctypes.c_int, [vp, ctypes.c_uint64]),
+    "wahgpu_server_wait": (ctypes.c_int, [vp, ctypes.c_uint64]),
+    "wahgpu_server_drain": (ctypes.c_int, [vp]),
+    "wahgpu_server_get_stats": (ctypes.c_int, [vp, ctypes.POINTER(ServerStats)]),
+    "wahgpu_server_reset_stats": (ctypes.c_int, [vp]),
     "wahgpu_sort_rids_by_value": (ctypes.c_int, [vp, vp, ctypes.c_uint64, ctypes.c_uint32,
                                                  vp, vp]),
     "wahgpu_produce_chunk_id_literals": (ctypes.c_int, [vp, vp, ctypes.c_uint64, vp, vp]),

@@ -13,7 +13,7 @@ namespace wah {
 
 // Tuning knobs (compile-time; defaults are the measured best on MI355X).
 #ifndef WAH_SORT_THREADS
-#define WAH_SORT_THREADS 512
+#define WAH_SORT_THREADS 1024
 #endif
 #ifndef WAH_SORT_ITEMS
 #define WAH_SORT_ITEMS 16

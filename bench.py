@@ -20,8 +20,8 @@ Beside it:
                 uniform half Zipf(1.0), seed 7), resident in HBM: one batched
                 build (graph replay) against a loop of per-column
                 wahgpu_build_device calls over the same columns
-  serve         requests drawn from the same distributions (seed 11), host
-                memory in and out: the continuous-batching server against the
+  serve         requests drawn from the same distributions (seed 11), page-
+                locked host memory in and out: the continuous-batching server against the
                 plain per-column drop-in (wahgpu_build per request), first as
                 a burst (saturation throughput), then as an open-loop Poisson
                 stream at half the plain path's saturation rate (p50 / p99)
@@ -144,20 +144,26 @@ def bench_serve(nreq, seed=11, arena_gib=64):
     from indexing_amd import workloads as W
     specs = W.column_specs(nreq, seed)
     progress(f"serve: generating {nreq} requests")
-    buf, layout = W.make_columns(specs)
+    # requests and results live in page-locked host memory (both paths)
+    buf, layout = W.make_columns(specs, alloc=ia.pinned_empty)
     cols = [buf[o:o + n] for o, n in layout]
     total = int(sum(n for _, n in layout))
+    span = buf.size
+    okeys, ooffs, olens, owords = (ia.pinned_empty(span), ia.pinned_empty(span),
+                                   ia.pinned_empty(span), ia.pinned_empty(2 * span))
+    outs = [(okeys[o:o + n], ooffs[o:o + n], olens[o:o + n], owords[2 * o:2 * (o + n)])
+            for o, n in layout]
 
     def run(srv, arrivals):
         progress(f"serve: {len(arrivals)} requests")
         srv.reset_stats()
         t0 = time.perf_counter()
         pend = []
-        for c, at in zip(cols, arrivals):
+        for c, o, at in zip(cols, outs, arrivals):
             delay = t0 + at - time.perf_counter()
             if delay > 0:
                 time.sleep(delay)
-            pend.append(srv.submit(c))
+            pend.append(srv.submit(c, out=o))
         for p in pend:
             srv.wait(p)
         st = srv.stats()
@@ -183,7 +189,7 @@ def bench_serve(nreq, seed=11, arena_gib=64):
     res.update({"requests": nreq, "values": total, "rate_req_s": round(rate, 1),
                 "workload": f"{nreq} requests, lengths log-uniform [2^12, 2^24], cardinality "
                             "log-uniform [2, 2^16], half uniform half Zipf(1.0), seed 11, "
-                            f"host memory, {arena_gib} GiB arena, 2 slots",
+                            f"page-locked host memory in and out, {arena_gib} GiB arena, 2 slots",
                 "throughput_ratio_burst": round(res["served_burst"]["gvalues_s"] /
                                                 res["plain_burst"]["gvalues_s"], 2)})
     return res

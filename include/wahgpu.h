@@ -233,6 +233,12 @@ int wahgpu_server_get_stats(wahgpu_server* s, wahgpu_server_stats* out);
 /* Clears the latency record and counters (between measurement phases). */
 int wahgpu_server_reset_stats(wahgpu_server* s);
 
+/* Page-locked host memory (hipHostMalloc) for request values and result
+ * buffers: copies from and to it run at full link speed and asynchronously,
+ * where pageable memory is staged through a bounce buffer. */
+int wahgpu_host_alloc(uint64_t bytes, void** out);
+void wahgpu_host_free(void* p);
+
 /* ---- stages (device pointers; each synchronizes when it returns a count) */
 /* include/sort_rids_by_value.cl:24-30 + include/radix.cl:65-233: stable sort
  * of (value, rid = rid_base + i) by value. */

@@ -21,7 +21,7 @@ from . import _native as N
 from ._native import WahError, MAX_VALUES
 
 __all__ = [
-    "Context", "Server", "Index", "WahError", "MAX_VALUES", "Merger", "read_values_text",
+    "Context", "Server", "Index", "pinned_empty", "WahError", "MAX_VALUES", "Merger", "read_values_text",
     "read_values_binary", "decode_bitmap", "generate", "device_count",
 ]
 
@@ -433,6 +433,31 @@ class Merger:
         return Index(keys[:K], offs[:K], lens[:K], words[:L], 0, L, K)
 
 
+class _PinnedBlock:
+    """Owns one wahgpu_host_alloc block; freed with the last array on it."""
+
+    def __init__(self, nbytes):
+        self.ptr = ctypes.c_void_p()
+        rc = N.lib().wahgpu_host_alloc(int(nbytes), ctypes.byref(self.ptr))
+        if rc != N.OK:
+            raise WahError(rc, f"pinned allocation of {nbytes} bytes failed")
+
+    def __del__(self):
+        if self.ptr:
+            N.lib().wahgpu_host_free(self.ptr)
+            self.ptr = None
+
+
+def pinned_empty(n, dtype=np.uint32):
+    """Uninitialised numpy array in page-locked host memory."""
+    dt = np.dtype(dtype)
+    nbytes = max(int(n), 1) * dt.itemsize
+    blk = _PinnedBlock(nbytes)
+    raw = (ctypes.c_uint8 * nbytes).from_address(blk.ptr.value)
+    raw._block = blk  # the ctypes view keeps the block alive for the array
+    return np.frombuffer(raw, dtype=dt)[:int(n)]
+
+
 class _Pending:
     """A submitted request: keeps the values and result buffers alive."""
     __slots__ = ("id", "values", "bufs", "res")
@@ -472,17 +497,27 @@ class Server:
     def __exit__(self, *exc):
         self.close()
 
-    def submit(self, values, rid_base=0):
+    def submit(self, values, rid_base=0, out=None):
+        """Queue one column.  out = (keys, offsets, lengths, words) result
+        arrays (n, n, n, 2n uint32; pinned_empty ones copy fastest), or None
+        for fresh pageable arrays."""
         p = _Pending()
         p.values = _as_u32(values)
         cap = max(p.values.size, 1)
-        p.bufs = (np.empty(cap, np.uint32), np.empty(cap, np.uint32), np.empty(cap, np.uint32),
-                  np.empty(2 * cap, np.uint32))
+        if out is None:
+            p.bufs = (np.empty(cap, np.uint32), np.empty(cap, np.uint32),
+                      np.empty(cap, np.uint32), np.empty(2 * cap, np.uint32))
+        else:
+            p.bufs = tuple(out)
+            if (any(b.dtype != np.uint32 or not b.flags.c_contiguous for b in p.bufs)
+                    or min(b.size for b in p.bufs[:3]) < p.values.size
+                    or p.bufs[3].size < 2 * p.values.size):
+                raise ValueError("out needs contiguous uint32 arrays of n, n, n and 2n entries")
         r = N.Result()
         r.keys, r.offsets, r.lengths = (x.ctypes.data for x in p.bufs[:3])
-        r.keys_cap = cap
+        r.keys_cap = min(b.size for b in p.bufs[:3])
         r.words = p.bufs[3].ctypes.data
-        r.words_cap = 2 * cap
+        r.words_cap = p.bufs[3].size
         p.res = r
         rid = ctypes.c_uint64(0)
         rc = self._lib.wahgpu_server_submit(self._h, p.values.ctypes.data_as(ctypes.c_void_p),

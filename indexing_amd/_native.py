@@ -109,6 +109,8 @@ SIGNATURES = {
     "wahgpu_server_drain": (ctypes.c_int, [vp]),
     "wahgpu_server_get_stats": (ctypes.c_int, [vp, ctypes.POINTER(ServerStats)]),
     "wahgpu_server_reset_stats": (ctypes.c_int, [vp]),
+    "wahgpu_host_alloc": (ctypes.c_int, [ctypes.c_uint64, ctypes.POINTER(vp)]),
+    "wahgpu_host_free": (None, [vp]),
     "wahgpu_sort_rids_by_value": (ctypes.c_int, [vp, vp, ctypes.c_uint64, ctypes.c_uint32,
                                                  vp, vp]),
     "wahgpu_produce_chunk_id_literals": (ctypes.c_int, [vp, vp, ctypes.c_uint64, vp, vp]),

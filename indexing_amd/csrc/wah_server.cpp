@@ -356,6 +356,21 @@ int wahgpu_server_get_stats(wahgpu_server* s, wahgpu_server_stats* out) {
   return WAHGPU_OK;
 }
 
+int wahgpu_host_alloc(uint64_t bytes, void** out) {
+  if (!out) return WAHGPU_EINVAL;
+  *out = nullptr;
+  if (hipHostMalloc(out, bytes ? bytes : 1, hipHostMallocDefault) != hipSuccess) {
+    (void)hipGetLastError();
+    *out = nullptr;
+    return WAHGPU_ENOMEM;
+  }
+  return WAHGPU_OK;
+}
+
+void wahgpu_host_free(void* p) {
+  if (p) (void)hipHostFree(p);
+}
+
 int wahgpu_server_reset_stats(wahgpu_server* s) {
   if (!s) return WAHGPU_EINVAL;
   std::lock_guard<std::mutex> lk(s->mu);

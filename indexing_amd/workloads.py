@@ -34,14 +34,15 @@ def make_column(spec, out=None):
     return generate(n, "zipf", seed=cseed, cardinality=card, zipf_s=1.0, out=out)
 
 
-def make_columns(specs, align=64, threads=None):
-    """Generates every column into one buffer at `align`-value boundaries.
+def make_columns(specs, align=64, threads=None, alloc=None):
+    """Generates every column into one buffer at `align`-value boundaries
+    (alloc(n) makes the buffer, e.g. pinned_empty; default numpy zeros).
     -> (buffer, [(offset, n)])."""
     layout, pos = [], 0
     for n, *_ in specs:
         layout.append((pos, n))
         pos += (n + align - 1) // align * align
-    buf = np.zeros(max(pos, 1), np.uint32)
+    buf = alloc(max(pos, 1)) if alloc else np.zeros(max(pos, 1), np.uint32)
     threads = threads or min(16, os.cpu_count() or 1)
     with cf.ThreadPoolExecutor(threads) as ex:
         list(ex.map(lambda a: make_column(a[0], out=buf[a[1][0]:a[1][0] + a[1][1]]),

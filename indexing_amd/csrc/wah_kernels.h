@@ -19,7 +19,7 @@ namespace wah {
 #define WAH_SORT_ITEMS 16
 #endif
 #ifndef WAH_TAIL_THREADS
-#define WAH_TAIL_THREADS 512
+#define WAH_TAIL_THREADS 1024
 #endif
 #ifndef WAH_TAIL_ITEMS
 #define WAH_TAIL_ITEMS 8

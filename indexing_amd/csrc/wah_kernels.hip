@@ -404,8 +404,22 @@ __device__ __forceinline__ SortedView sorted_view(const ColState* col, const Col
 // the packed (keys << 16 | words) counts, then a wave-parallel look-back
 // across tiles.  Words are staged in LDS and stored contiguously.
 // ---------------------------------------------------------------------------
+// Bit sums (positions 0-15 and 16-30 apart) of the lanes before the first
+// entry start of 64 consecutive pairs; zero when lane 0 starts an entry.
+// b = the lane's rid % 31 (31: no pair), h = the lane starts an entry.
+__device__ __forceinline__ void run_lead(uint32_t b, bool h, uint32_t* lo, uint32_t* hi) {
+  const uint32_t blo = b < 16 ? 1u << b : 0u;
+  const uint32_t bhi = (b >= 16 && b < 31) ? 1u << (b - 16) : 0u;
+  const uint32_t slo = wave_incl_add(blo), shi = wave_incl_add(bhi);
+  const uint64_t hm = ballot(h);
+  const int fh = hm ? __builtin_ctzll(hm) : kWave;
+  const bool cont = (hm & 1ull) == 0;
+  *lo = cont ? (uint32_t)__builtin_amdgcn_readlane((int)slo, fh - 1) : 0u;
+  *hi = cont ? (uint32_t)__builtin_amdgcn_readlane((int)shi, fh - 1) : 0u;
+}
+
 template <int THREADS, int ITEMS>
-__global__ __launch_bounds__(THREADS) void k_tail(
+__global__ __launch_bounds__(THREADS, 4) void k_tail(
     const DescHead* __restrict__ desc, const ColState* __restrict__ cs, Header* __restrict__ hdr,
     const uint32_t* values, const uint32_t* keysA, const uint32_t* ridsA,
     const uint32_t* keysB, const uint32_t* ridsB, uint32_t* __restrict__ keys_out,
@@ -420,6 +434,7 @@ __global__ __launch_bounds__(THREADS) void k_tail(
   static_assert(TILE % 4 == 0 && TILE <= 32768, "word counts fit 16 bits");
   __shared__ __align__(16) uint32_t s_buf[2 * SPAN];
   __shared__ uint32_t s_wtot[WAVES];
+  __shared__ uint32_t s_lead[2 * (WAVES + 1)];
   __shared__ uint64_t s_prefix;
   __shared__ uint64_t s_ent[WAVES];
   __shared__ uint32_t s_tile;
@@ -501,7 +516,10 @@ __global__ __launch_bounds__(THREADS) void k_tail(
     }
     __syncthreads();
 
-    uint32_t fill[ITEMS], lit[ITEMS], key[ITEMS], ex[ITEMS];
+    // Pass 1 over the wave's pairs: entry and key starts, fills, word and
+    // key counts.  bit[i] keeps the pair's literal bit (rid % 31, or 31 for
+    // none) for the literals, which pass 2 forms with prefix sums.
+    uint32_t fill[ITEMS], lit[ITEMS], key[ITEMS], ex[ITEMS], bit[ITEMS];
     uint32_t khmask = 0;
     uint32_t carry = 0;  // (keys << 16 | words) of this wave's earlier rows
 #pragma unroll
@@ -512,21 +530,16 @@ __global__ __launch_bounds__(THREADS) void k_tail(
       fill[i] = 0;
       lit[i] = 0;
       key[i] = 0;
+      bit[i] = 31;
       uint32_t p = 0;
       if (g < n) {
         const uint32_t v = s_k[s], r = s_r[s], c = r / 31u;
         const bool first = g == 0;
         const uint32_t cp = first ? 0u : s_r[s - 1] / 31u;
         const bool kh = first || v != s_k[s - 1];
+        bit[i] = r % 31u;
         if (kh || c != cp) {
-          uint32_t x = 0x80000000u | (1u << (r % 31u));
-          for (int m = 1; m < 31; ++m) {
-            if (g + m >= n) break;
-            const uint32_t r2 = s_r[s + m];
-            if (s_k[s + m] != v || r2 / 31u != c) break;
-            x |= 1u << (r2 % 31u);
-          }
-          lit[i] = x;
+          lit[i] = 0x80000000u;  // entry start; payload bits added in pass 2
           fill[i] = kh ? c : c - cp - 1u;
           p = 1u + (fill[i] != 0u);
           ++entries;
@@ -542,7 +555,72 @@ __global__ __launch_bounds__(THREADS) void k_tail(
       carry += readlane(inc, kWave - 1);
     }
     if (l == 0) s_wtot[w] = carry;
+    // Literal bits of the entry run that continues into this wave's first
+    // 64 pairs from the previous wave (and, for the last wave, into the
+    // pairs past the tile): the bit sum of the pairs before the first entry
+    // start.  A run has at most 31 pairs (one chunk), so it ends inside the
+    // next 64.
+    {
+      uint32_t b0 = bit[0];
+      bool h0 = lit[0] != 0u;
+      uint32_t lo, hi;
+      run_lead(b0, h0, &lo, &hi);
+      if (l == 0) {
+        s_lead[2 * w] = lo;
+        s_lead[2 * w + 1] = hi;
+      }
+      if (w == WAVES - 1) {
+        const int j = TILE + l;
+        const uint64_t g = base + j;
+        uint32_t bh = 31;
+        bool hh = false;
+        if (l < HALO && g < n) {
+          const int sh = PAD + j;
+          const uint32_t r = s_r[sh];
+          bh = r % 31u;
+          hh = s_k[sh] != s_k[sh - 1] || r / 31u != s_r[sh - 1] / 31u;
+        } else {
+          hh = true;  // nothing continues past the column
+        }
+        run_lead(bh, hh, &lo, &hi);
+        if (l == 0) {
+          s_lead[2 * WAVES] = lo;
+          s_lead[2 * WAVES + 1] = hi;
+        }
+      }
+    }
     __syncthreads();
+    // Pass 2: literal of each entry start = OR of the bits of its run.  The
+    // bits of a run are distinct, so the OR is a sum: difference of wave
+    // prefix sums (low and high 16 bit positions summed apart) from the start
+    // to the lane before the next start, plus the continuation into the next
+    // 64 pairs when the run reaches lane 63.
+    {
+      uint32_t nlo = s_lead[2 * (w + 1)], nhi = s_lead[2 * (w + 1) + 1];
+#pragma unroll
+      for (int i = ITEMS - 1; i >= 0; --i) {
+        const uint32_t b = bit[i];
+        const uint32_t blo = b < 16 ? 1u << b : 0u;
+        const uint32_t bhi = (b >= 16 && b < 31) ? 1u << (b - 16) : 0u;
+        const uint32_t slo = wave_incl_add(blo), shi = wave_incl_add(bhi);
+        const uint64_t hm = ballot(lit[i] != 0u);
+        const uint64_t rest = l == kWave - 1 ? 0ull : (hm >> (l + 1));
+        const int nh = rest ? l + 1 + __builtin_ctzll(rest) : kWave;
+        const uint32_t elo = (uint32_t)__shfl((int)slo, nh - 1, kWave);
+        const uint32_t ehi = (uint32_t)__shfl((int)shi, nh - 1, kWave);
+        uint32_t lo = elo - (slo - blo), hi = ehi - (shi - bhi);
+        if (nh == kWave) {
+          lo += nlo;
+          hi += nhi;
+        }
+        if (lit[i]) lit[i] = 0x80000000u | (hi << 16) | lo;
+        // this item's leading continuation, for item i - 1
+        const int fh = hm ? __builtin_ctzll(hm) : kWave;
+        const bool cont = (hm & 1ull) == 0;
+        nlo = cont ? (uint32_t)__builtin_amdgcn_readlane((int)slo, fh - 1) : 0u;
+        nhi = cont ? (uint32_t)__builtin_amdgcn_readlane((int)shi, fh - 1) : 0u;
+      }
+    }
     uint32_t woff = 0, tot = 0;
 #pragma unroll
     for (int i = 0; i < WAVES; ++i) {

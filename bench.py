@@ -201,6 +201,9 @@ def main():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--n", type=int, default=N_VALUES, help="column length (default 2^28)")
+    ap.add_argument("--column", choices=["uniform", "zipf-runs"], default="uniform",
+                    help="uniform [0,65535] seed 1 (default), or Zipf(1.2) over 2^20 values in "
+                         "runs of mean length 4096, seed 1")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--batch-cols", type=int, default=1024,
                     help="columns of the batched workload (0 skips it)")
@@ -218,7 +221,13 @@ def main():
     import indexing_amd as ia
 
     n = args.n
-    host = ia.generate(n, "uniform", seed=1, lo=0, hi=65535)
+    if args.column == "uniform":
+        host = ia.generate(n, "uniform", seed=1, lo=0, hi=65535)
+        column = "uniform [0,65535], seed 1"
+    else:
+        host = ia.generate(n, "zipf-runs", seed=1, cardinality=1 << 20, zipf_s=1.2,
+                           run_mean=4096.0)
+        column = "Zipf(1.2) over 2^20 values in runs of mean length 4096, seed 1"
     d_values = torch.from_numpy(host.view(np.int32)).to("cuda:0")
     ctx = ia.Context(0)
     out = ctx.build_device(d_values)                   # allocates outputs + arena
@@ -246,12 +255,15 @@ def main():
     pass_ms = [group_ms[1 + p] / calls for p in range(4)]
     # the P active passes are the P slowest (inactive ones exit at once)
     active = sorted(sorted(range(4), key=lambda p: -pass_ms[p])[:P])
-    pass_bytes = [12.0 * n] + [16.0 * n] * (P - 1)         # per active launch, in order
+    if counts.mode == 0:
+        pass_bytes = [12.0 * n] + [16.0 * n] * (P - 1)     # per active launch, in order
+    else:  # run mode: launch 0 reads the rows, every launch moves 12-byte run items (>= k)
+        pass_bytes = [4.0 * n + 12.0 * counts.k] + [24.0 * counts.k] * (P - 1)
     sort_ms = sum(pass_ms[p] for p in active)
     per_launch_bytes = sum(pass_bytes) / P
     per_launch_ms = sort_ms / P
     achieved = per_launch_bytes / (per_launch_ms * 1e-3) / 1e9
-    traffic = read_traffic()
+    traffic = read_traffic() if args.column == "uniform" and n == N_VALUES else None
     b_alg = 16.0 * n * P + 8.0 * n + 4.0 * L + 12.0 * K
     build_ms_events = sum(group_ms) / calls
 
@@ -267,11 +279,12 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "u32",
-        "data": "synthetic (seeded splitmix64 uniform column, generated on the host)",
+        "data": "synthetic (seeded splitmix64 column, generated on the host)",
         "config": {"workload": f"single-column WAH build, n=2^{n.bit_length() - 1} uint32 "
-                               "uniform [0,65535], seed 1, resident in HBM",
-                   "n": n, "k": int(counts.k), "L": L, "K": K, "radix_passes": P},
-        "roofline": {"bound": "hbm", "kernel": "k_onesweep (radix pass)",
+                               f"{column}, resident in HBM",
+                   "n": n, "k": int(counts.k), "L": L, "K": K, "radix_passes": P,
+                   "sort_mode": "runs" if counts.mode else "pairs"},
+        "roofline": {"bound": "hbm", "kernel": "k_onesweep (radix pass, mean of active launches)",
                      "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": traffic,

@@ -56,7 +56,10 @@ typedef struct wahgpu_counts {
   uint32_t fault;   /* nonzero: a kernel gave up waiting on a look-back
                        status or caught an out-of-range index; the
                        outputs are not valid (never expected)            */
-  uint32_t reserved;
+  uint32_t mode;    /* sort mode of the fused build: 0 = one (value, rid)
+                       pair per row, 1 = one item per run of equal values
+                       inside a 31-row chunk (chosen when that sorts under
+                       2/3 of the rows); staged paths report 0           */
 } wahgpu_counts;
 
 /* Microseconds per phase, the buckets `phases` prints (src/phases.cpp:714-1127). */

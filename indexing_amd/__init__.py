@@ -52,6 +52,7 @@ class Index:
     bound: int = 0
     passes: int = 0
     times: dict = field(default_factory=dict)
+    mode: int = 0  # 0: one (value, rid) pair per row sorted; 1: one item per run
 
     def key_words(self, i):
         o = int(self.offsets[i])
@@ -159,7 +160,7 @@ class Context:
         K, L = int(c.keycnt), int(c.length)
         times = {name: getattr(t, name) for name, _ in N.PhaseTimes._fields_} if timed else {}
         return Index(keys[:K].copy(), offs[:K].copy(), lens[:K].copy(), words[:L].copy(),
-                     int(c.k), L, K, int(c.bound), int(c.passes), times)
+                     int(c.k), L, K, int(c.bound), int(c.passes), times, int(c.mode))
 
     def build_device(self, d_values, rid_base=0, out=None):
         """Fused build on a device tensor; returns (keys, offsets, lengths, words,
@@ -222,7 +223,7 @@ class Context:
             K, L = int(c.keycnt), int(c.length)
             k, o, ln, w = bufs[i]
             out.append(Index(k[:K].copy(), o[:K].copy(), ln[:K].copy(), w[:L].copy(), int(c.k),
-                             L, K, int(c.bound), int(c.passes)))
+                             L, K, int(c.bound), int(c.passes), mode=int(c.mode)))
         return out
 
     def build_batch_device(self, d_values, columns, out=None):
@@ -532,7 +533,8 @@ class Server:
         c = p.res.counts
         K, L = int(c.keycnt), int(c.length)
         k, o, ln, w = p.bufs
-        return Index(k[:K], o[:K], ln[:K], w[:L], int(c.k), L, K, int(c.bound), int(c.passes))
+        return Index(k[:K], o[:K], ln[:K], w[:L], int(c.k), L, K, int(c.bound), int(c.passes),
+                     mode=int(c.mode))
 
     def poll(self, p):
         """-> Index when done, None while pending (raises on failure)."""

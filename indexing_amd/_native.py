@@ -27,7 +27,7 @@ class Counts(ctypes.Structure):
     _fields_ = [("k", ctypes.c_uint64), ("length", ctypes.c_uint64),
                 ("keycnt", ctypes.c_uint64), ("bound", ctypes.c_uint32),
                 ("passes", ctypes.c_uint32), ("fault", ctypes.c_uint32),
-                ("reserved", ctypes.c_uint32)]
+                ("mode", ctypes.c_uint32)]
 
 
 class PhaseTimes(ctypes.Structure):

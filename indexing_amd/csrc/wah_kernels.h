@@ -37,8 +37,13 @@ constexpr int kTailTile = kTailThreads * kTailItems;  // sorted pairs per tail t
 constexpr int kScanItems = 8;
 constexpr int kScanTile = 256 * kScanItems;
 constexpr uint32_t kHistMaxBlocks = 2048;
-constexpr int kCounters = 8;
+// Tile counters: radix launches 0-3 of pair-mode columns, then their tail;
+// radix launches 0-3 of run-mode columns, then their tail.
+constexpr int kCounters = 11;
 constexpr int kCounterTail = 4;
+constexpr int kCounterRun = 5;
+constexpr int kCounterRunTail = 9;
+constexpr int kCounterCut = 10;
 
 // Which buffer a pass reads / writes.
 constexpr uint32_t kBufValues = 0;  // the input column, rids implicit (rid_base + i)
@@ -82,32 +87,46 @@ __host__ __device__ inline Column* desc_cols(DescHead* d) {
 struct Header {
   uint32_t tiles[kCounters][kLaneWords];  // tile-counter lanes per kernel
   uint32_t fault;                         // nonzero after a failed wait / bad index
-  uint32_t max_active;                    // most radix passes any column needs
-  uint32_t pad[62];
+  uint32_t max_active[2];                 // most radix passes of a column, per mode
+  uint32_t mode_cols[2];                  // nonempty columns per mode
+  uint32_t pad[59];
 };
 
 // Per-column state.  hist and maxval are zeroed by k_begin; the rest is
 // written by k_plan.
+// Sort modes of a column.  Pair mode sorts one (value, rid) pair per row.
+// Run mode first cuts the column into runs of equal values inside one
+// 31-row chunk and sorts one item per run: (value, rid of its first row,
+// literal bits of its rows); columns whose rows repeat their predecessor
+// (runs, low cardinality in order) shrink to a fraction of n items.
+constexpr uint32_t kModePair = 0;
+constexpr uint32_t kModeRun = 1;
+
 struct ColState {
   uint32_t hist[4][256];          // histogram of each 8-bit digit
+  uint32_t rep[4][256];           // the same over rows that repeat their predecessor
   uint32_t maxval;
   uint32_t num_active;            // radix passes this column needs
-  uint32_t final_buf;             // kBuf* holding its sorted pairs
+  uint32_t final_buf;             // kBuf* holding its sorted items
   uint32_t shift[4];              // digit shift of its a-th active pass
-  uint32_t pad[57];
-  uint32_t digit_offset[4][256];  // exclusive prefix of each digit histogram
+  uint32_t mode;                  // kModePair / kModeRun
+  uint32_t items;                 // items sorted: n, or the runs in run mode
+  uint32_t pad[55];
+  uint32_t digit_offset[4][256];  // exclusive prefix of each item digit histogram
 };
-constexpr uint32_t kColZeroWords = 4 * 256 + 1;
+constexpr uint32_t kColZeroWords = 8 * 256 + 1;
 
 struct BuildArgs {
   const uint32_t* values;            // concatenated input
   uint32_t *keysA, *ridsA, *keysB, *ridsB;
+  uint32_t *litsA, *litsB;           // run mode: literal bits of each item
   DescHead* desc;                    // device descriptor
   Header* hdr;
   ColState* cs;                      // cols_cap entries
   uint32_t* sort_status;             // 2 arrays of sort_status_stride words
   uint64_t sort_status_stride;
   uint64_t* tail_status;             // tail_tiles_cap words
+  uint64_t* cut_status;              // sort_tiles_cap words (run cut look-back)
   uint32_t *out_keys, *out_offsets, *out_lengths, *out_words;
   uint32_t *sorted_keys, *sorted_rids;  // stage API sort output (one column)
   wahgpu_counts_dev* counts;         // per column
@@ -116,6 +135,8 @@ struct BuildArgs {
   // within them; the kernels read the actual sizes from the descriptor).
   uint32_t cols_cap, sort_tiles_cap, tail_tiles_cap, hist_chunks_cap;
   uint64_t n_cap;
+  // Every column sorts (value, rid) pairs (the stage API needs them).
+  bool force_pair;
   // Single column: k_begin writes the descriptor from these (no host copy).
   bool inline_desc;
   Dims one_dims;

@@ -69,8 +69,33 @@ __device__ __forceinline__ void hist_add(uint32_t* h, uint32_t bin, bool valid) 
   }
 }
 
-__device__ __forceinline__ void hist_flush(uint32_t* h, uint32_t* s_max, uint32_t& vmax,
-                                           ColState* col) {
+// Adds a value to the digit histograms, and to the repeat histograms when
+// it repeats its predecessor inside one chunk (rep).
+__device__ __forceinline__ void hist_value(uint32_t* h, uint32_t* hr, uint32_t v, bool valid,
+                                           bool rep) {
+  const uint32_t f = __builtin_amdgcn_readfirstlane(v);
+  if (ballot(valid && v != f) == 0) {  // one value across the wave (runs)
+    const uint32_t cnt = (uint32_t)__popcll(ballot(valid));
+    const uint32_t rc = (uint32_t)__popcll(ballot(rep));
+    if (lane_id() == 0 && cnt) {
+#pragma unroll
+      for (int p = 0; p < 4; ++p) {
+        atomicAdd(&h[p * 256 + ((f >> (8 * p)) & 255u)], cnt);
+        if (rc) atomicAdd(&hr[p * 256 + ((f >> (8 * p)) & 255u)], rc);
+      }
+    }
+    return;
+  }
+#pragma unroll
+  for (int p = 0; p < 4; ++p) hist_add(h, p * 256 + ((v >> (8 * p)) & 255u), valid);
+  if (ballot(rep)) {
+#pragma unroll
+    for (int p = 0; p < 4; ++p) hist_add(hr, p * 256 + ((v >> (8 * p)) & 255u), rep);
+  }
+}
+
+__device__ __forceinline__ void hist_flush(uint32_t* h, uint32_t* hr, uint32_t* s_max,
+                                           uint32_t& vmax, ColState* col) {
   const int t = threadIdx.x;
   for (int d = 32; d >= 1; d >>= 1) {
     const uint32_t o = __shfl_xor(vmax, d, kWave);
@@ -80,7 +105,9 @@ __device__ __forceinline__ void hist_flush(uint32_t* h, uint32_t* s_max, uint32_
   __syncthreads();
   for (int i = t; i < 1024; i += kBlock) {
     if (h[i]) atomicAdd(&col->hist[0][i], h[i]);
+    if (hr[i]) atomicAdd(&col->rep[0][i], hr[i]);
     h[i] = 0;
+    hr[i] = 0;
   }
   if (t == 0) {
     uint32_t m = 0;
@@ -94,19 +121,21 @@ __device__ __forceinline__ void hist_flush(uint32_t* h, uint32_t* s_max, uint32_
 __global__ __launch_bounds__(kBlock) void k_histogram(
     const uint32_t* __restrict__ values, const DescHead* __restrict__ desc,
     ColState* __restrict__ cs, uint32_t* __restrict__ status0,
-    uint64_t* __restrict__ tail_status) {
+    uint64_t* __restrict__ tail_status, uint64_t* __restrict__ cut_status) {
   __shared__ uint32_t h[4 * 256];
+  __shared__ uint32_t hr[4 * 256];
   __shared__ uint32_t s_max[kWaves];
   const int t = threadIdx.x;
   const Dims dims = desc->dims;
   const Column* cols = desc_cols(desc);
-  for (int i = t; i < 1024; i += kBlock) h[i] = 0;
+  for (int i = t; i < 1024; i += kBlock) h[i] = hr[i] = 0;
   // zero look-back state used later in this build
   const uint64_t gthreads = (uint64_t)gridDim.x * kBlock;
   const uint64_t gt = (uint64_t)blockIdx.x * kBlock + t;
   const uint64_t s0 = (uint64_t)dims.sort_tiles * 256;
   for (uint64_t i = gt; i < s0; i += gthreads) status0[i] = 0;
   for (uint64_t i = gt; i < dims.tail_tiles; i += gthreads) tail_status[i] = 0;
+  for (uint64_t i = gt; i < dims.sort_tiles; i += gthreads) cut_status[i] = 0;
   __syncthreads();
 
   const uint32_t per = (dims.hist_chunks + gridDim.x - 1) / gridDim.x;
@@ -118,7 +147,7 @@ __global__ __launch_bounds__(kBlock) void k_histogram(
   uint32_t vmax = 0;
   for (uint32_t c = c0; c < c1; ++c) {
     if (c >= col_end) {  // next column (block-uniform)
-      hist_flush(h, s_max, vmax, &cs[col]);
+      hist_flush(h, hr, s_max, vmax, &cs[col]);
       col = find_segment(&cols[0].hist_chunk0, sizeof(Column) / 4, dims.ncols, c);
       col_end = col + 1 < dims.ncols ? cols[col + 1].hist_chunk0 : dims.hist_chunks;
     }
@@ -126,6 +155,9 @@ __global__ __launch_bounds__(kBlock) void k_histogram(
     const uint64_t lo = (uint64_t)(c - cd.hist_chunk0) * kHistChunk;
     const uint32_t cnt = cd.n - lo < kHistChunk ? (uint32_t)(cd.n - lo) : kHistChunk;
     const uint32_t* src = values + cd.off + lo;
+    // row of src[i] is rid_base + lo + i; a row repeats its predecessor when
+    // it is not the first of the column or of a chunk and has its value
+    const uint32_t row0 = cd.rid_base + (uint32_t)lo;
     if (cnt == kHistChunk && ((uintptr_t)src % 16) == 0) {
       // four 16-byte loads per thread
       const uint4* v4 = reinterpret_cast<const uint4*>(src);
@@ -134,12 +166,40 @@ __global__ __launch_bounds__(kBlock) void k_histogram(
       for (int u = 0; u < 4; ++u) q[u] = v4[u * kBlock + t];
 #pragma unroll
       for (int u = 0; u < 4; ++u) {
+        const uint32_t i0 = (uint32_t)(u * kBlock + t) * 4;  // index of q[u].x in src
+        uint32_t prev = (uint32_t)__shfl_up((int)q[u].w, 1, kWave);
+        if (lane_id() == 0) prev = (lo + i0 > 0) ? src[(int64_t)i0 - 1] : 0u;
         const uint32_t e[4] = {q[u].x, q[u].y, q[u].z, q[u].w};
+        bool rep[4];
 #pragma unroll
         for (int j = 0; j < 4; ++j) {
           vmax = e[j] > vmax ? e[j] : vmax;
+          const uint32_t pv = j == 0 ? prev : e[j - 1];
+          rep[j] = lo + i0 + j > 0 && (row0 + i0 + j) % 31u != 0u && e[j] == pv;
+        }
+        // one value in all 256 rows of the wave (runs): one add per bin
+        const uint32_t f = __builtin_amdgcn_readfirstlane(e[0]);
+        if (ballot(e[0] != f || e[1] != f || e[2] != f || e[3] != f) == 0) {
+          uint32_t rc = 0;
 #pragma unroll
-          for (int p = 0; p < 4; ++p) hist_add(h, p * 256 + ((e[j] >> (8 * p)) & 255u), true);
+          for (int j = 0; j < 4; ++j) rc += (uint32_t)__popcll(ballot(rep[j]));
+          if (lane_id() == 0) {
+#pragma unroll
+            for (int p = 0; p < 4; ++p) {
+              atomicAdd(&h[p * 256 + ((f >> (8 * p)) & 255u)], 4u * kWave);
+              if (rc) atomicAdd(&hr[p * 256 + ((f >> (8 * p)) & 255u)], rc);
+            }
+          }
+        } else {
+#pragma unroll
+          for (int j = 0; j < 4; ++j) {
+#pragma unroll
+            for (int p = 0; p < 4; ++p) hist_add(h, p * 256 + ((e[j] >> (8 * p)) & 255u), true);
+            if (ballot(rep[j])) {
+#pragma unroll
+              for (int p = 0; p < 4; ++p) hist_add(hr, p * 256 + ((e[j] >> (8 * p)) & 255u), rep[j]);
+            }
+          }
         }
       }
     } else {
@@ -147,13 +207,14 @@ __global__ __launch_bounds__(kBlock) void k_histogram(
         const uint32_t i = b + t;
         const bool valid = i < cnt;
         const uint32_t v = valid ? src[i] : 0;
+        const bool has_prev = valid && lo + i > 0;
+        const uint32_t pv = has_prev ? src[(int64_t)i - 1] : 0u;
         vmax = valid && v > vmax ? v : vmax;
-#pragma unroll
-        for (int p = 0; p < 4; ++p) hist_add(h, p * 256 + ((v >> (8 * p)) & 255u), valid);
+        hist_value(h, hr, v, valid, has_prev && (row0 + i) % 31u != 0u && v == pv);
       }
     }
   }
-  hist_flush(h, s_max, vmax, &cs[col]);
+  hist_flush(h, hr, s_max, vmax, &cs[col]);
 }
 
 // After the histogram, per column: digit offsets and the pass plan (digits
@@ -162,22 +223,32 @@ __global__ __launch_bounds__(kBlock) void k_histogram(
 // each one writes back and invalidates the L2).
 __global__ __launch_bounds__(kBlock) void k_plan(const DescHead* __restrict__ desc,
                                                  ColState* __restrict__ cs, Header* __restrict__ hdr,
-                                                 wahgpu_counts_dev* __restrict__ counts) {
+                                                 wahgpu_counts_dev* __restrict__ counts,
+                                                 int force_pair) {
   __shared__ uint64_t s_scan[kWaves];
   __shared__ int s_trivial[4];
+  __shared__ uint32_t s_reps;
   const int t = threadIdx.x;
   const uint32_t ncols = desc->dims.ncols;
   for (uint32_t c = blockIdx.x; c < ncols; c += gridDim.x) {
     const uint64_t n = desc_cols(desc)[c].n;
     ColState* col = &cs[c];
     if (t < 4) s_trivial[t] = 0;
+    if (t == 0) s_reps = 0;
     __syncthreads();
+    atomicAdd(&s_reps, col->rep[0][t]);
+    __syncthreads();
+    // Run mode when it sorts under 2/3 of the rows (an item moves 12 bytes
+    // per pass, a pair 8).
+    const uint64_t runs = n - s_reps;
+    const bool run_mode = !force_pair && n > 0 && 3 * runs < 2 * n;
+    const uint64_t items = run_mode ? runs : n;
     for (int p = 0; p < 4; ++p) {
-      const uint32_t h = col->hist[p][t];
+      const uint32_t h = col->hist[p][t] - (run_mode ? col->rep[p][t] : 0u);
       uint64_t total;
       const uint64_t ex = block_exclusive<OpAdd>(h, s_scan, &total);
       col->digit_offset[p][t] = (uint32_t)ex;
-      if ((uint64_t)h == n) s_trivial[p] = 1;  // every value has digit t
+      if ((uint64_t)h == items) s_trivial[p] = 1;  // every item has digit t
     }
     __syncthreads();
     if (t == 0) {
@@ -186,11 +257,16 @@ __global__ __launch_bounds__(kBlock) void k_plan(const DescHead* __restrict__ de
         if (n > 0 && !s_trivial[p]) col->shift[a++] = 8u * p;
       for (uint32_t q = a; q < 4; ++q) col->shift[q] = 0;
       col->num_active = a;
-      col->final_buf = (a == 0) ? kBufValues : ((a - 1) % 2 == 0 ? kBufA : kBufB);
-      if (a) atomicMax(&hdr->max_active, a);
+      col->mode = run_mode ? kModeRun : kModePair;
+      col->items = (uint32_t)items;
+      // pair mode: values, then A, B, A, B; run mode: the cut writes B
+      if (run_mode) col->final_buf = (a % 2) ? kBufA : kBufB;
+      else col->final_buf = (a == 0) ? kBufValues : ((a - 1) % 2 == 0 ? kBufA : kBufB);
+      if (a) atomicMax(&hdr->max_active[run_mode ? 1 : 0], a);
+      if (n) atomicAdd(&hdr->mode_cols[run_mode ? 1 : 0], 1u);
       wahgpu_counts_dev z;
       z.k = 0; z.length = 0; z.keycnt = 0;
-      z.bound = col->maxval; z.passes = a; z.fault = 0; z.reserved = 0;
+      z.bound = col->maxval; z.passes = a; z.fault = 0; z.mode = run_mode ? kModeRun : kModePair;
       counts[c] = z;
     }
     __syncthreads();
@@ -206,52 +282,189 @@ __global__ __launch_bounds__(kBlock) void k_plan(const DescHead* __restrict__ de
 // (stability).  Keys and then rids are staged in one LDS buffer in digit
 // order and written out in runs per digit.
 // ---------------------------------------------------------------------------
+// Run-mode helpers for launch 0, which reads the rows in order: a row starts
+// a run unless it repeats its predecessor's value inside the same chunk.
+// stop = lanes that start a run or hold no row; a run's length is the
+// distance to the next stop, continuing into the next 64 rows (lead = their
+// leading non-stop lanes) when it reaches lane 63.
+__device__ __forceinline__ uint32_t run_length(uint64_t stop, uint32_t l, uint32_t lead) {
+  const uint64_t rest = l == kWave - 1 ? 0ull : (stop >> (l + 1));
+  return rest ? 1u + (uint32_t)__builtin_ctzll(rest) : (uint32_t)kWave - l + lead;
+}
+
+// Run mode, before the radix launches: cuts each run-mode column into runs
+// (a row starts one unless it repeats its predecessor's value inside the
+// same chunk) and writes one item per run, in row order, to buffer B:
+// value, rid of its first row, literal bits of its rows.  An ordered stream
+// compaction: per-wave ballots, a block scan and a look-back over one count
+// per tile; tiles are the radix tiles of the column.
 template <int THREADS, int ITEMS>
+__global__ __launch_bounds__(THREADS) void k_runcut(
+    const uint32_t* __restrict__ values, uint32_t* __restrict__ keysB,
+    uint32_t* __restrict__ ridsB, uint32_t* __restrict__ litsB,
+    const DescHead* __restrict__ desc, const ColState* __restrict__ cs,
+    Header* __restrict__ hdr, uint64_t* cut_status, uint32_t lanes) {
+  constexpr int TILE = THREADS * ITEMS;
+  constexpr int WAVES = THREADS / kWave;
+  __shared__ uint32_t s_lead[WAVES + 1];  // leading non-stop lanes of each wave's first 64 rows
+  __shared__ uint32_t s_wsum[WAVES];
+  __shared__ uint64_t s_prefix;
+  __shared__ uint32_t s_tile;
+  const Dims dims = desc->dims;
+  const Column* cols = desc_cols(desc);
+  const int t = threadIdx.x, w = t / kWave, l = t % kWave;
+  if (hdr->mode_cols[1] == 0) return;  // no run-mode column
+  uint32_t iter = 0;
+  for (;;) {
+    const uint32_t tile = __builtin_amdgcn_readfirstlane(
+        next_tile(hdr->tiles[kCounterCut], lanes, &s_tile, iter));
+    if (tile >= dims.sort_tiles) break;
+    const uint32_t c = find_segment(&cols[0].sort_tile0, sizeof(Column) / 4, dims.ncols, tile);
+    const ColState* col = &cs[c];
+    if (col->mode != kModeRun) {
+      __syncthreads();  // s_tile is read by every wave before it is rewritten
+      continue;
+    }
+    const Column cd = cols[c];
+    const uint64_t n = cd.n;
+    const uint64_t base = (uint64_t)(tile - cd.sort_tile0) * TILE;
+    const uint64_t wbase = base + (uint64_t)w * kWave * ITEMS + l;
+    const uint32_t* vp = values + cd.off;
+    uint32_t key[ITEMS];
+    uint64_t startm[ITEMS], stop[ITEMS];
+#pragma unroll
+    for (int i = 0; i < ITEMS; ++i) {
+      const uint64_t pos = wbase + (uint64_t)(i * kWave);
+      key[i] = pos < n ? vp[pos] : 0u;
+    }
+    uint32_t mine = 0;  // run starts of this wave
+#pragma unroll
+    for (int i = 0; i < ITEMS; ++i) {
+      const uint64_t pos = wbase + (uint64_t)(i * kWave);
+      const bool valid = pos < n;
+      uint32_t prev = (uint32_t)__shfl_up((int)key[i], 1, kWave);
+      if (i > 0) {
+        const uint32_t last = readlane(key[i > 0 ? i - 1 : 0], kWave - 1);
+        if (l == 0) prev = last;
+      } else if (l == 0 && valid && pos > 0) {
+        prev = vp[pos - 1];
+      }
+      const bool start = valid && (pos == 0 || (cd.rid_base + (uint32_t)pos) % 31u == 0u ||
+                                   key[i] != prev);
+      startm[i] = ballot(start);
+      stop[i] = ballot(start || !valid);
+      mine += (uint32_t)__popcll(startm[i]);
+    }
+    if (l == 0) {
+      s_lead[w] = (uint32_t)__builtin_ctzll(stop[0]);
+      s_wsum[w] = mine;
+    }
+    if (w == WAVES - 1) {  // the rows past the tile: where its last run ends
+      const uint64_t pos = base + TILE + l;
+      const bool valid = pos < n;
+      const uint32_t v = valid ? vp[pos] : 0u;
+      uint32_t prev = (uint32_t)__shfl_up((int)v, 1, kWave);
+      const uint32_t last = readlane(key[ITEMS - 1], kWave - 1);
+      if (l == 0) prev = last;
+      const bool start = valid && ((cd.rid_base + (uint32_t)pos) % 31u == 0u || v != prev);
+      const uint64_t m = ballot(start || !valid);
+      if (l == 0) s_lead[WAVES] = (uint32_t)__builtin_ctzll(m);
+    }
+    __syncthreads();
+    uint32_t before = 0, total = 0;
+#pragma unroll
+    for (int i = 0; i < WAVES; ++i) {
+      before += i < w ? s_wsum[i] : 0u;
+      total += s_wsum[i];
+    }
+    if (w == 0) {  // wave-parallel look-back: 64 predecessors per round trip
+      const uint64_t pre = lookback_wave_sum(cut_status, tile, cd.sort_tile0, total, &hdr->fault);
+      if (l == 0) s_prefix = pre;
+    }
+    __syncthreads();
+    uint64_t idx = s_prefix + before;
+    const uint64_t items = col->items;
+#pragma unroll
+    for (int i = 0; i < ITEMS; ++i) {
+      if ((startm[i] >> l) & 1ull) {
+        const uint32_t lead = i + 1 < ITEMS ? (uint32_t)__builtin_ctzll(stop[i + 1 < ITEMS ? i + 1 : i])
+                                            : s_lead[w + 1];
+        const uint32_t len = run_length(stop[i], (uint32_t)l, lead);
+        const uint32_t row = cd.rid_base + (uint32_t)(wbase + (uint64_t)(i * kWave));
+        const uint64_t j = idx + mbcnt(startm[i]);
+        if (j < items) {
+          keysB[cd.off + j] = key[i];
+          ridsB[cd.off + j] = row;
+          litsB[cd.off + j] = ((len >= 32u ? 0u : (1u << len)) - 1u) << (row % 31u);
+        } else {
+          atomicOr(&hdr->fault, 2u);
+        }
+      }
+      idx += (uint64_t)__popcll(startm[i]);
+    }
+    __syncthreads();
+  }
+}
+
+template <int THREADS, int ITEMS, bool RUN>
 __global__ __launch_bounds__(THREADS, WAH_SORT_WAVES_PER_EU) void k_onesweep(
     uint32_t a, const uint32_t* __restrict__ values, uint32_t* keysA, uint32_t* ridsA,
-    uint32_t* keysB, uint32_t* ridsB, const DescHead* __restrict__ desc,
-    const ColState* __restrict__ cs, Header* __restrict__ hdr, uint32_t* status_base,
-    uint64_t status_stride, uint32_t lanes) {
+    uint32_t* keysB, uint32_t* ridsB, uint32_t* litsA, uint32_t* litsB,
+    const DescHead* __restrict__ desc, const ColState* __restrict__ cs, Header* __restrict__ hdr,
+    uint32_t* status_base, uint64_t status_stride, uint32_t lanes) {
   constexpr int TILE = THREADS * ITEMS;
   constexpr int WAVES = THREADS / kWave;
   static_assert(THREADS >= 256 && THREADS % 256 == 0, "one thread per digit");
+  static_assert(ITEMS <= 32, "participation bits");
+  static_assert(TILE <= 65536, "LDS slots packed in 16 bits");
   __shared__ uint32_t s_stage[TILE];
   __shared__ uint8_t s_digit[TILE];
   __shared__ uint32_t s_wcnt[WAVES][256];
   __shared__ uint32_t s_excl[256];
   __shared__ uint32_t s_glob[256];
   __shared__ uint32_t s_dsum[4];
+  __shared__ uint32_t s_tilen;
   __shared__ uint32_t s_tile;
 
-  const bool implicit = a == 0;
+  // pair mode: launch 0 reads the values (rids implicit); run mode: the run
+  // cut wrote the items to B.  Then A -> B, B -> A, ...
+  const bool implicit = !RUN && a == 0;
   const uint32_t* src_keys = implicit ? values : ((a % 2) ? keysA : keysB);
   const uint32_t* src_rids = (a % 2) ? ridsA : ridsB;
+  const uint32_t* src_lits = (a % 2) ? litsA : litsB;
   uint32_t* dst_keys = (a % 2) ? keysB : keysA;
   uint32_t* dst_rids = (a % 2) ? ridsB : ridsA;
+  uint32_t* dst_lits = (a % 2) ? litsB : litsA;
   uint32_t* status = status_base + (uint64_t)(a % 2) * status_stride;
   uint32_t* status_next = status_base + (uint64_t)(1u - a % 2) * status_stride;
   uint32_t* fault = &hdr->fault;
   const Dims dims = desc->dims;
   const Column* cols = desc_cols(desc);
   const int t = threadIdx.x, w = t / kWave, l = t % kWave;
-  if (hdr->max_active <= a) return;  // no column has an a-th pass
+  if (hdr->max_active[RUN ? 1 : 0] <= a) return;  // no column has an a-th pass
+  uint32_t* counter = hdr->tiles[(RUN ? kCounterRun : 0) + a];
 
   uint32_t iter = 0;
   for (;;) {
     for (int i = t; i < WAVES * 256; i += THREADS) (&s_wcnt[0][0])[i] = 0;
-    const uint32_t tile = __builtin_amdgcn_readfirstlane(next_tile(hdr->tiles[a], lanes, &s_tile, iter));
+    const uint32_t tile = __builtin_amdgcn_readfirstlane(next_tile(counter, lanes, &s_tile, iter));
     if (tile >= dims.sort_tiles) break;
     const uint32_t c = find_segment(&cols[0].sort_tile0, sizeof(Column) / 4, dims.ncols, tile);
     const ColState* col = &cs[c];
-    if (col->num_active <= a) {  // this column is already sorted
-      __syncthreads();           // s_tile is read by every wave before it is rewritten
-      continue;
-    }
     const Column cd = cols[c];
     const uint32_t first = cd.sort_tile0;
-    const uint32_t shift = col->shift[a];
-    const uint64_t n = cd.n;
     const uint64_t base = (uint64_t)(tile - first) * TILE;  // column-local
+    // elements: rows, or the run items of a run-mode column
+    const uint64_t n = RUN ? (uint64_t)col->items : cd.n;
+    const uint64_t n_out = n;
+    if (col->mode != (RUN ? kModeRun : kModePair) || col->num_active <= a || base >= n) {
+      // other mode, already sorted, or past the run items; a block draws its
+      // tiles in increasing order, so past the last column's items it is done
+      if (RUN && c + 1 == dims.ncols && col->mode == kModeRun && base >= n) break;
+      __syncthreads();  // s_tile is read by every wave before it is rewritten
+      continue;
+    }
+    const uint32_t shift = col->shift[a];
     if (t < 256) status_next[(uint64_t)tile * 256 + t] = 0;  // next launch's state
 
     // Keys only for now: the rids are loaded once the keys are staged, so
@@ -270,11 +483,14 @@ __global__ __launch_bounds__(THREADS, WAH_SORT_WAVES_PER_EU) void k_onesweep(
         for (int i = 0; i < ITEMS; ++i) key[i] = (uint32_t)(i * kWave) < nvalid ? kp[i * kWave] : 0u;
       }
     }
+    uint32_t part = 0;  // items this lane ranks
+#pragma unroll
+    for (int i = 0; i < ITEMS; ++i) part |= ((uint32_t)(i * kWave) < nvalid ? 1u : 0u) << i;
     // Rank: peers = lanes with the same digit (8 ballots); rank = peers
     // below this lane + keys of that digit the wave ranked before.
 #pragma unroll
     for (int i = 0; i < ITEMS; ++i) {
-      if ((uint32_t)(i * kWave) < nvalid) {
+      if ((part >> i) & 1u) {
         const uint32_t d = (key[i] >> shift) & 255u;
         uint64_t peers = ballot(true);
 #pragma unroll
@@ -293,9 +509,9 @@ __global__ __launch_bounds__(THREADS, WAH_SORT_WAVES_PER_EU) void k_onesweep(
     __syncthreads();
     // Threads 0..255 own one digit each: wave prefixes, tile total,
     // exclusive scan over digits, look-back.
+    uint32_t total = 0;
     if (t < 256) {
       const uint32_t goff = col->digit_offset[shift / 8][t];
-      uint32_t total = 0;
 #pragma unroll
       for (int i = 0; i < WAVES; ++i) {
         const uint32_t x = s_wcnt[i][t];
@@ -322,11 +538,12 @@ __global__ __launch_bounds__(THREADS, WAH_SORT_WAVES_PER_EU) void k_onesweep(
       const uint32_t ex = s_excl[t] + before;
       s_excl[t] = ex;
       s_glob[t] -= ex;  // column index = s_glob[d] + local index
+      if (t == 255) s_tilen = ex + total;
     }
     __syncthreads();
 #pragma unroll
     for (int i = 0; i < ITEMS; ++i) {
-      if ((uint32_t)(i * kWave) < nvalid) {
+      if ((part >> i) & 1u) {
         const uint32_t d = (key[i] >> shift) & 255u;
         const uint32_t pos = s_excl[d] + s_wcnt[w][d] + rank[i];
         s_stage[pos] = key[i];
@@ -334,6 +551,11 @@ __global__ __launch_bounds__(THREADS, WAH_SORT_WAVES_PER_EU) void k_onesweep(
         rank[i] = pos;  // keep the LDS slot for the rids
       }
     }
+    // LDS slots, two per register (a tile has at most 2^16 items)
+    uint32_t slot2[(ITEMS + 1) / 2];
+#pragma unroll
+    for (int i = 0; i < ITEMS; i += 2)
+      slot2[i / 2] = (rank[i] & 0xFFFFu) | (i + 1 < ITEMS ? rank[i + 1 < ITEMS ? i + 1 : i] << 16 : 0u);
     // rids: issued now, in flight while the keys go out
     uint32_t rid[ITEMS];
     if (implicit) {
@@ -345,7 +567,7 @@ __global__ __launch_bounds__(THREADS, WAH_SORT_WAVES_PER_EU) void k_onesweep(
       for (int i = 0; i < ITEMS; ++i) rid[i] = (uint32_t)(i * kWave) < nvalid ? rp[i * kWave] : 0u;
     }
     __syncthreads();
-    const uint32_t tile_n = (n - base) < (uint64_t)TILE ? (uint32_t)(n - base) : (uint32_t)TILE;
+    const uint32_t tile_n = s_tilen;
     uint32_t* dk = dst_keys + cd.off;
     uint32_t* dr = dst_rids + cd.off;
 #pragma unroll
@@ -353,21 +575,39 @@ __global__ __launch_bounds__(THREADS, WAH_SORT_WAVES_PER_EU) void k_onesweep(
       const uint32_t q = (uint32_t)j * THREADS + t;
       if (q < tile_n) {
         const uint32_t g = s_glob[s_digit[q]] + q;
-        if (g < n) dk[g] = s_stage[q];
+        if (g < n_out) dk[g] = s_stage[q];
         else atomicOr(fault, 2u);
       }
     }
     __syncthreads();
 #pragma unroll
     for (int i = 0; i < ITEMS; ++i)
-      if ((uint32_t)(i * kWave) < nvalid) s_stage[rank[i]] = rid[i];
+      if ((part >> i) & 1u) s_stage[(slot2[i / 2] >> (16 * (i % 2))) & 0xFFFFu] = rid[i];
     __syncthreads();
 #pragma unroll
     for (int j = 0; j < ITEMS; ++j) {
       const uint32_t q = (uint32_t)j * THREADS + t;
       if (q < tile_n) {
         const uint32_t g = s_glob[s_digit[q]] + q;
-        if (g < n) dr[g] = s_stage[q];
+        if (g < n_out) dr[g] = s_stage[q];
+      }
+    }
+    if (RUN) {
+      // literal bits of each run item, moved along with its key and rid
+      __syncthreads();
+      const uint32_t* lp = src_lits + cd.off + wbase;  // one base, immediate offsets
+#pragma unroll
+      for (int i = 0; i < ITEMS; ++i)
+        if ((part >> i) & 1u) s_stage[(slot2[i / 2] >> (16 * (i % 2))) & 0xFFFFu] = lp[i * kWave];
+      __syncthreads();
+      uint32_t* dl = dst_lits + cd.off;
+#pragma unroll 4  // a full unroll keeps the slot indices of all three rounds live
+      for (int j = 0; j < ITEMS; ++j) {
+        const uint32_t q = (uint32_t)j * THREADS + t;
+        if (q < tile_n) {
+          const uint32_t g = s_glob[s_digit[q]] + q;
+          if (g < n_out) dl[g] = s_stage[q];
+        }
       }
     }
     __syncthreads();
@@ -379,38 +619,29 @@ __global__ __launch_bounds__(THREADS, WAH_SORT_WAVES_PER_EU) void k_onesweep(
 struct SortedView {
   const uint32_t* keys;
   const uint32_t* rids;
+  const uint32_t* lits;  // run mode
   bool implicit;
 };
 __device__ __forceinline__ SortedView sorted_view(const ColState* col, const Column& cd,
                                                   const uint32_t* values,
                                                   const uint32_t* keysA, const uint32_t* ridsA,
-                                                  const uint32_t* keysB, const uint32_t* ridsB) {
+                                                  const uint32_t* keysB, const uint32_t* ridsB,
+                                                  const uint32_t* litsA = nullptr,
+                                                  const uint32_t* litsB = nullptr) {
   SortedView v;
   const uint32_t f = col->final_buf;
   v.implicit = f == kBufValues;
   v.keys = (f == kBufValues ? values : (f == kBufA ? keysA : keysB)) + cd.off;
   v.rids = (f == kBufA ? ridsA : ridsB) + cd.off;
+  v.lits = (f == kBufA ? litsA : litsB) + cd.off;
   return v;
 }
 
-// ---------------------------------------------------------------------------
-// Fused tail over the sorted (value, rid) pairs.  A tile of THREADS x ITEMS
-// pairs is staged in LDS; wave w owns pairs [w*64*ITEMS, +64*ITEMS) in (item,
-// lane) order, so lanes read consecutive LDS words.  Per pair: chid = rid/31;
-// an entry starts where (value, chid) changes and ORs the literal bits of its
-// run (at most 31 pairs, read ahead in LDS); a key starts where value
-// changes.  fill = chid for a key start, else chid - chid_prev - 1; an entry
-// emits [fill if nonzero, literal].  Word and key positions: DPP wave scans of
-// the packed (keys << 16 | words) counts, then a wave-parallel look-back
-// across tiles.  Words are staged in LDS and stored contiguously.
-// ---------------------------------------------------------------------------
 // Bit sums (positions 0-15 and 16-30 apart) of the lanes before the first
-// entry start of 64 consecutive pairs; zero when lane 0 starts an entry.
-// b = the lane's rid % 31 (31: no pair), h = the lane starts an entry.
+// entry start of 64 consecutive items; zero when lane 0 starts an entry.
+// b = the lane's literal bits (0: no item), h = the lane starts an entry.
 __device__ __forceinline__ void run_lead(uint32_t b, bool h, uint32_t* lo, uint32_t* hi) {
-  const uint32_t blo = b < 16 ? 1u << b : 0u;
-  const uint32_t bhi = (b >= 16 && b < 31) ? 1u << (b - 16) : 0u;
-  const uint32_t slo = wave_incl_add(blo), shi = wave_incl_add(bhi);
+  const uint32_t slo = wave_incl_add(b & 0xFFFFu), shi = wave_incl_add(b >> 16);
   const uint64_t hm = ballot(h);
   const int fh = hm ? __builtin_ctzll(hm) : kWave;
   const bool cont = (hm & 1ull) == 0;
@@ -418,13 +649,26 @@ __device__ __forceinline__ void run_lead(uint32_t b, bool h, uint32_t* lo, uint3
   *hi = cont ? (uint32_t)__builtin_amdgcn_readlane((int)shi, fh - 1) : 0u;
 }
 
-template <int THREADS, int ITEMS>
+// ---------------------------------------------------------------------------
+// Fused tail over the sorted items: (value, rid) pairs, or in run mode
+// (value, rid, literal bits) runs.  A tile of THREADS x ITEMS items is staged
+// in LDS; wave w owns items [w*64*ITEMS, +64*ITEMS) in (item, lane) order, so
+// lanes read consecutive LDS words.  Per item: chid = rid/31; an entry starts
+// where (value, chid) changes and ORs the literal bits of the items up to
+// the next entry (at most 31 rows); a key starts where value changes.
+// fill = chid for a key start, else chid - chid_prev - 1; an entry emits
+// [fill if nonzero, literal].  Word and key positions: DPP wave scans of the
+// packed (keys << 16 | words) counts, then a wave-parallel look-back across
+// tiles.  Words are staged in LDS and stored contiguously.
+// ---------------------------------------------------------------------------
+template <int THREADS, int ITEMS, bool RUN>
 __global__ __launch_bounds__(THREADS, 4) void k_tail(
     const DescHead* __restrict__ desc, const ColState* __restrict__ cs, Header* __restrict__ hdr,
     const uint32_t* values, const uint32_t* keysA, const uint32_t* ridsA,
-    const uint32_t* keysB, const uint32_t* ridsB, uint32_t* __restrict__ keys_out,
-    uint32_t* __restrict__ offsets_out, uint32_t* __restrict__ words_out, uint64_t* status,
-    uint32_t lanes, wahgpu_counts_dev* __restrict__ counts) {
+    const uint32_t* keysB, const uint32_t* ridsB, const uint32_t* litsA, const uint32_t* litsB,
+    uint32_t* __restrict__ keys_out, uint32_t* __restrict__ offsets_out,
+    uint32_t* __restrict__ words_out, uint64_t* status, uint32_t lanes,
+    wahgpu_counts_dev* __restrict__ counts) {
   constexpr int TILE = THREADS * ITEMS;
   constexpr int WAVES = THREADS / kWave;
   constexpr int SUB = kWave * ITEMS;     // pairs per wave
@@ -432,7 +676,7 @@ __global__ __launch_bounds__(THREADS, 4) void k_tail(
   constexpr int HALO = 32;               // pairs read past the tile by runs
   constexpr int SPAN = TILE + PAD + HALO;
   static_assert(TILE % 4 == 0 && TILE <= 32768, "word counts fit 16 bits");
-  __shared__ __align__(16) uint32_t s_buf[2 * SPAN];
+  __shared__ __align__(16) uint32_t s_buf[(RUN ? 3 : 2) * SPAN];
   __shared__ uint32_t s_wtot[WAVES];
   __shared__ uint32_t s_lead[2 * (WAVES + 1)];
   __shared__ uint64_t s_prefix;
@@ -440,13 +684,15 @@ __global__ __launch_bounds__(THREADS, 4) void k_tail(
   __shared__ uint32_t s_tile;
   uint32_t* s_k = s_buf;
   uint32_t* s_r = s_buf + SPAN;
-  uint32_t* s_words = s_buf;  // reused once the pairs are consumed (2 * TILE words)
+  uint32_t* s_l = s_buf + 2 * SPAN;  // run mode: literal bits
+  uint32_t* s_words = s_buf;  // reused once the items are consumed (2 * TILE words)
   const Dims dims = desc->dims;
   const Column* cols = desc_cols(desc);
   const int t = threadIdx.x, w = t / kWave, l = t % kWave;
   uint32_t* fault = &hdr->fault;
   uint64_t entries = 0;           // entries of column ent_col counted by this thread
   uint32_t ent_col = 0xFFFFFFFFu;
+  if (hdr->mode_cols[RUN ? 1 : 0] == 0) return;  // no column in this mode
   // k of a column: one atomic per block and column
   auto flush_entries = [&]() {
     const uint64_t e = wave_sum64(entries);
@@ -462,24 +708,33 @@ __global__ __launch_bounds__(THREADS, 4) void k_tail(
 
   uint32_t iter = 0;
   for (;;) {
-    const uint32_t tile = __builtin_amdgcn_readfirstlane(next_tile(&hdr->tiles[kCounterTail][0], lanes, &s_tile, iter));
+    const uint32_t tile = __builtin_amdgcn_readfirstlane(next_tile(
+        &hdr->tiles[RUN ? kCounterRunTail : kCounterTail][0], lanes, &s_tile, iter));
     if (tile >= dims.tail_tiles) break;
     const uint32_t c = find_segment(&cols[0].tail_tile0, sizeof(Column) / 4, dims.ncols, tile);
+    const Column cd = cols[c];
+    const uint32_t first_tile = cd.tail_tile0;
+    const uint64_t base = (uint64_t)(tile - first_tile) * TILE;  // column-local
+    // items: rows, or the runs of a run-mode column (fewer than its tiles)
+    const uint64_t n = RUN ? (uint64_t)cs[c].items : cd.n;
+    if (cs[c].mode != (RUN ? kModeRun : kModePair) || base >= n) {
+      if (RUN && c + 1 == dims.ncols && cs[c].mode == kModeRun) break;  // past the last items
+      __syncthreads();  // s_tile is read by every wave before it is rewritten
+      continue;
+    }
     if (c != ent_col) {  // block-uniform
       if (ent_col != 0xFFFFFFFFu) flush_entries();
       ent_col = c;
     }
-    const Column cd = cols[c];
-    const uint64_t n = cd.n;
     const uint32_t rid_base = cd.rid_base;
-    const uint32_t first_tile = cd.tail_tile0;
-    const SortedView sv = sorted_view(&cs[c], cd, values, keysA, ridsA, keysB, ridsB);
+    const SortedView sv = sorted_view(&cs[c], cd, values, keysA, ridsA, keysB, ridsB, litsA,
+                                      litsB);
     uint32_t* out_keys = keys_out + cd.off;
     uint32_t* out_offsets = offsets_out + cd.off;
     uint32_t* out_words = words_out + 2 * cd.off;
-    const uint64_t base = (uint64_t)(tile - first_tile) * TILE;  // column-local
     // stage pairs [base - 1, base + TILE + HALO - 1) with 16-byte loads
-    const bool aligned = (((uintptr_t)(sv.keys + base) | (uintptr_t)(sv.rids + base)) % 16) == 0;
+    const bool aligned = (((uintptr_t)(sv.keys + base) | (uintptr_t)(sv.rids + base) |
+                           (RUN ? (uintptr_t)(sv.lits + base) : (uintptr_t)0)) % 16) == 0;
     if (base + TILE <= n && aligned) {
       const uint4* k4 = reinterpret_cast<const uint4*>(sv.keys + base);
       const uint4* r4 = reinterpret_cast<const uint4*>(sv.rids + base);
@@ -493,6 +748,7 @@ __global__ __launch_bounds__(THREADS, 4) void k_tail(
           rr = r4[j];
         }
         reinterpret_cast<uint4*>(s_r + PAD)[j] = rr;
+        if (RUN) reinterpret_cast<uint4*>(s_l + PAD)[j] = reinterpret_cast<const uint4*>(sv.lits + base)[j];
       }
     } else {
       for (int j = t; j < TILE; j += THREADS) {
@@ -500,6 +756,7 @@ __global__ __launch_bounds__(THREADS, 4) void k_tail(
         if (g < n) {
           s_k[PAD + j] = sv.keys[g];
           s_r[PAD + j] = sv.implicit ? rid_base + (uint32_t)g : sv.rids[g];
+          if (RUN) s_l[PAD + j] = sv.lits[g];
         }
       }
     }
@@ -508,6 +765,7 @@ __global__ __launch_bounds__(THREADS, 4) void k_tail(
       if (g < n) {
         s_k[PAD + TILE + t] = sv.keys[g];
         s_r[PAD + TILE + t] = sv.implicit ? rid_base + (uint32_t)g : sv.rids[g];
+        if (RUN) s_l[PAD + TILE + t] = sv.lits[g];
       }
     }
     if (t == THREADS - 1 && base > 0) {
@@ -516,9 +774,9 @@ __global__ __launch_bounds__(THREADS, 4) void k_tail(
     }
     __syncthreads();
 
-    // Pass 1 over the wave's pairs: entry and key starts, fills, word and
-    // key counts.  bit[i] keeps the pair's literal bit (rid % 31, or 31 for
-    // none) for the literals, which pass 2 forms with prefix sums.
+    // Pass 1 over the wave's items: entry and key starts, fills, word and
+    // key counts.  bit[i] keeps the item's literal bits (1 << rid % 31 for a
+    // pair; 0 for none) for the literals, which pass 2 forms with prefix sums.
     uint32_t fill[ITEMS], lit[ITEMS], key[ITEMS], ex[ITEMS], bit[ITEMS];
     uint32_t khmask = 0;
     uint32_t carry = 0;  // (keys << 16 | words) of this wave's earlier rows
@@ -530,14 +788,14 @@ __global__ __launch_bounds__(THREADS, 4) void k_tail(
       fill[i] = 0;
       lit[i] = 0;
       key[i] = 0;
-      bit[i] = 31;
+      bit[i] = 0;
       uint32_t p = 0;
       if (g < n) {
         const uint32_t v = s_k[s], r = s_r[s], c = r / 31u;
         const bool first = g == 0;
         const uint32_t cp = first ? 0u : s_r[s - 1] / 31u;
         const bool kh = first || v != s_k[s - 1];
-        bit[i] = r % 31u;
+        bit[i] = RUN ? s_l[s] : 1u << (r % 31u);
         if (kh || c != cp) {
           lit[i] = 0x80000000u;  // entry start; payload bits added in pass 2
           fill[i] = kh ? c : c - cp - 1u;
@@ -572,12 +830,12 @@ __global__ __launch_bounds__(THREADS, 4) void k_tail(
       if (w == WAVES - 1) {
         const int j = TILE + l;
         const uint64_t g = base + j;
-        uint32_t bh = 31;
+        uint32_t bh = 0;
         bool hh = false;
         if (l < HALO && g < n) {
           const int sh = PAD + j;
           const uint32_t r = s_r[sh];
-          bh = r % 31u;
+          bh = RUN ? s_l[sh] : 1u << (r % 31u);
           hh = s_k[sh] != s_k[sh - 1] || r / 31u != s_r[sh - 1] / 31u;
         } else {
           hh = true;  // nothing continues past the column
@@ -599,9 +857,7 @@ __global__ __launch_bounds__(THREADS, 4) void k_tail(
       uint32_t nlo = s_lead[2 * (w + 1)], nhi = s_lead[2 * (w + 1) + 1];
 #pragma unroll
       for (int i = ITEMS - 1; i >= 0; --i) {
-        const uint32_t b = bit[i];
-        const uint32_t blo = b < 16 ? 1u << b : 0u;
-        const uint32_t bhi = (b >= 16 && b < 31) ? 1u << (b - 16) : 0u;
+        const uint32_t blo = bit[i] & 0xFFFFu, bhi = bit[i] >> 16;
         const uint32_t slo = wave_incl_add(blo), shi = wave_incl_add(bhi);
         const uint64_t hm = ballot(lit[i] != 0u);
         const uint64_t rest = l == kWave - 1 ? 0ull : (hm >> (l + 1));
@@ -991,17 +1247,29 @@ void launch_sort(const BuildArgs& a, hipStream_t s, LaunchHook* hook) {
   uint64_t hb = (a.n_cap + 4 * kHistChunk - 1) / (4 * kHistChunk);
   if (hb > a.hist_chunks_cap) hb = a.hist_chunks_cap;
   hipLaunchKernelGGL(k_histogram, dim3(cap_grid((uint32_t)hb, kHistMaxBlocks)), dim3(kBlock), 0,
-                     s, a.values, a.desc, a.cs, a.sort_status, a.tail_status);
+                     s, a.values, a.desc, a.cs, a.sort_status, a.tail_status, a.cut_status);
   hipLaunchKernelGGL(k_plan, dim3(cap_grid(a.cols_cap, 1024)), dim3(kBlock), 0, s, a.desc, a.cs,
-                     a.hdr, a.counts);
-  if (hook) hook->mark(s);
+                     a.hdr, a.counts, (int)a.force_pair);
   const uint32_t grid =
       cap_grid(a.sort_tiles_cap, WAH_STATIC_TILES ? a.sort_tiles_cap
                                                   : a.max_resident * 256 / kSortThreads);
+  // the cut needs half the registers of a radix pass: two blocks per CU
+  const uint32_t cut_grid = cap_grid(a.sort_tiles_cap, 2 * grid);
+  hipLaunchKernelGGL((k_runcut<kSortThreads, kSortItems>), dim3(cut_grid), dim3(kSortThreads), 0,
+                     s, a.values, a.keysB, a.ridsB, a.litsB, a.desc, a.cs, a.hdr, a.cut_status,
+                     tile_lanes(cut_grid));
+  if (hook) hook->mark(s);
+  // each launch runs the a-th pass of pair-mode columns, then of run-mode
+  // columns; a kernel with no such column returns at once
   for (uint32_t p = 0; p < 4; ++p) {
-    hipLaunchKernelGGL((k_onesweep<kSortThreads, kSortItems>), dim3(grid), dim3(kSortThreads), 0,
-                       s, p, a.values, a.keysA, a.ridsA, a.keysB, a.ridsB, a.desc, a.cs, a.hdr,
-                       a.sort_status, a.sort_status_stride, tile_lanes(grid));
+    hipLaunchKernelGGL((k_onesweep<kSortThreads, kSortItems, false>), dim3(grid),
+                       dim3(kSortThreads), 0, s, p, a.values, a.keysA, a.ridsA, a.keysB, a.ridsB,
+                       a.litsA, a.litsB, a.desc, a.cs, a.hdr, a.sort_status, a.sort_status_stride,
+                       tile_lanes(grid));
+    hipLaunchKernelGGL((k_onesweep<kSortThreads, kSortItems, true>), dim3(grid),
+                       dim3(kSortThreads), 0, s, p, a.values, a.keysA, a.ridsA, a.keysB, a.ridsB,
+                       a.litsA, a.litsB, a.desc, a.cs, a.hdr, a.sort_status, a.sort_status_stride,
+                       tile_lanes(grid));
     if (hook) hook->mark(s);
   }
 }
@@ -1010,10 +1278,14 @@ void launch_tail(const BuildArgs& a, hipStream_t s, LaunchHook* hook) {
   const uint32_t grid =
       cap_grid(a.tail_tiles_cap, WAH_STATIC_TILES ? a.tail_tiles_cap
                                                   : a.max_resident * 256 / kTailThreads);
-  hipLaunchKernelGGL((k_tail<kTailThreads, kTailItems>), dim3(grid), dim3(kTailThreads), 0, s,
-                     a.desc, a.cs, a.hdr, a.values, a.keysA, a.ridsA, a.keysB, a.ridsB,
-                     a.out_keys, a.out_offsets, a.out_words, a.tail_status, tile_lanes(grid),
-                     a.counts);
+  hipLaunchKernelGGL((k_tail<kTailThreads, kTailItems, false>), dim3(grid), dim3(kTailThreads),
+                     0, s, a.desc, a.cs, a.hdr, a.values, a.keysA, a.ridsA, a.keysB, a.ridsB,
+                     a.litsA, a.litsB, a.out_keys, a.out_offsets, a.out_words, a.tail_status,
+                     tile_lanes(grid), a.counts);
+  hipLaunchKernelGGL((k_tail<kTailThreads, kTailItems, true>), dim3(grid), dim3(kTailThreads),
+                     0, s, a.desc, a.cs, a.hdr, a.values, a.keysA, a.ridsA, a.keysB, a.ridsB,
+                     a.litsA, a.litsB, a.out_keys, a.out_offsets, a.out_words, a.tail_status,
+                     tile_lanes(grid), a.counts);
   if (hook) hook->mark(s);
   uint64_t lg = (a.n_cap + kBlock - 1) / kBlock;
   hipLaunchKernelGGL(k_finish, dim3(cap_grid(lg > 1024 ? 1024u : (uint32_t)lg, 1024)),

@@ -33,10 +33,10 @@ using Clock = std::chrono::steady_clock;
 
 // Device bytes a batch needs per value: the values, keys, offsets, lengths
 // and words (2 per value) outputs, the two radix ping-pong buffers of
-// (key, rid) pairs and the look-back state; and per column: its state block,
-// descriptor and tile rounding.
-constexpr uint64_t kBytesPerValue = 4 + 12 + 8 + 16 + 1;
-constexpr uint64_t kBytesPerColumn = 8448 + 32 + 3 * 8192;
+// (key, rid, literal) items and the look-back state; and per column: its
+// state block, descriptor and tile rounding.
+constexpr uint64_t kBytesPerValue = 4 + 12 + 8 + 24 + 1;
+constexpr uint64_t kBytesPerColumn = 12544 + 32 + 3 * 8192;
 constexpr uint64_t kSlotMargin = 64ull << 20;
 constexpr uint64_t kAlignValues = 64;  // columns start on 256-byte boundaries
 

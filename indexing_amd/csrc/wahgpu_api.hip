@@ -119,12 +119,13 @@ Caps caps_one(uint64_t n) {
 uint64_t build_scratch_bytes(const Caps& k) {
   const uint64_t nn = k.n ? k.n : 1;
   uint64_t b = 0;
-  b += 4 * align_up(nn * 4);                                       // keysA ridsA keysB ridsB
+  b += 6 * align_up(nn * 4);                       // keys, rids and lits of A and B
   b += align_up(sizeof(wah::DescHead) + (uint64_t)k.cols * sizeof(wah::Column));
   b += align_up(sizeof(wah::Header));
   b += align_up((uint64_t)k.cols * sizeof(wah::ColState));
   b += align_up(2ull * k.sort_tiles * 256 * 4);
   b += align_up((uint64_t)k.tail_tiles * 8 + 8);
+  b += align_up((uint64_t)k.sort_tiles * 8 + 8);
   return b;
 }
 
@@ -134,6 +135,8 @@ void carve_build(wahgpu_ctx* c, BuildArgs* a, const Caps& k) {
   a->ridsA = take<uint32_t>(c, nn);
   a->keysB = take<uint32_t>(c, nn);
   a->ridsB = take<uint32_t>(c, nn);
+  a->litsA = take<uint32_t>(c, nn);
+  a->litsB = take<uint32_t>(c, nn);
   a->desc = reinterpret_cast<wah::DescHead*>(
       take<char>(c, sizeof(wah::DescHead) + (uint64_t)k.cols * sizeof(wah::Column)));
   a->hdr = take<wah::Header>(c, 1);
@@ -141,6 +144,7 @@ void carve_build(wahgpu_ctx* c, BuildArgs* a, const Caps& k) {
   a->sort_status_stride = (uint64_t)k.sort_tiles * 256;
   a->sort_status = take<uint32_t>(c, 2 * a->sort_status_stride);
   a->tail_status = take<uint64_t>(c, (uint64_t)k.tail_tiles + 1);
+  a->cut_status = take<uint64_t>(c, (uint64_t)k.sort_tiles + 1);
   a->max_resident = c->max_resident;
   a->n_cap = k.n;
   a->cols_cap = k.cols;
@@ -790,6 +794,7 @@ int wahgpu_build_staged(wahgpu_ctx* c, const uint32_t* values, uint64_t n, uint3
   a.counts = d_counts;
   a.sorted_keys = s_keys;
   a.sorted_rids = s_rids;
+  a.force_pair = true;  // the stages below take one (value, rid) pair per row
   // The passes ping-pong between (keysA, ridsA) and (keysB, ridsB); finalize
   // copies whichever holds the result into (s_keys, s_rids).
   wah::launch_sort(a, c->stream);
@@ -831,6 +836,8 @@ int wahgpu_build_staged(wahgpu_ctx* c, const uint32_t* values, uint64_t n, uint3
   counts->keycnt = cc.keycnt;
   counts->bound = cc.bound;
   counts->passes = cc.passes;
+  counts->fault = 0;
+  counts->mode = 0;
   if (counts->keycnt > keys_cap || counts->length > words_cap)
     return fail(c, WAHGPU_EINVAL, "output capacity too small");
   if (L) HIPCHK(c, hipMemcpyAsync(words, index, L * 4, hipMemcpyDeviceToHost, c->stream));
@@ -875,6 +882,7 @@ int wahgpu_sort_rids_by_value(wahgpu_ctx* c, const uint32_t* d_values, uint64_t 
   a.counts = take<wahgpu_counts>(c, 1);
   a.sorted_keys = d_keys_out;
   a.sorted_rids = d_rids_out;
+  a.force_pair = true;
   wah::launch_sort(a, c->stream);
   wah::launch_sort_finalize(a, c->stream);
   LAUNCHCHK(c);

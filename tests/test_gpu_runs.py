@@ -1,0 +1,99 @@
+"""Run mode: columns whose rows repeat their predecessor are sorted as one item
+per run of equal values inside a 31-row chunk (value, first rid, literal
+bits).  The index must be the oracle's (src/cpu.cpp:313-328) bit for bit,
+whatever the runs look like at chunk, wave, tile and column boundaries.
+"""
+import numpy as np
+import pytest
+
+import indexing_amd as ia
+from oracle import oracle
+from tests.test_gpu_batch import pack, per_column
+from tests.test_gpu_parity import assert_same, dev
+
+pytestmark = pytest.mark.gpu
+
+
+def runs_column(n, mean, card, seed, dtype_max=1 << 16):
+    """Runs of geometric length with the given mean over `card` values."""
+    rng = np.random.default_rng(seed)
+    out = np.empty(n, np.uint32)
+    i = 0
+    while i < n:
+        ln = int(rng.geometric(1.0 / mean))
+        out[i:i + ln] = rng.integers(0, card) % dtype_max
+        i += ln
+    return out
+
+
+def cases():
+    rng = np.random.default_rng(5)
+    out = [
+        ("const_1", np.full(1, 4, np.uint32), 0),
+        ("const_31", np.full(31, 4, np.uint32), 0),
+        ("const_big", np.full(100000, 123456, np.uint32), 0),
+        ("two_vals", rng.integers(0, 2, 70000).astype(np.uint32), 0),
+        ("sorted", np.sort(rng.integers(0, 3000, 50001)).astype(np.uint32), 0),
+        ("runs_1", runs_column(40000, 1.5, 50, 1), 0),
+        ("runs_8", runs_column(16383, 8, 1000, 2), 0),
+        ("runs_31", runs_column(16384, 31, 1000, 3), 0),
+        ("runs_64", runs_column(16385, 64, 1 << 20, 4, 1 << 20), 0),
+        ("runs_1000", runs_column(300000, 1000, 1 << 30, 5, 1 << 31), 0),
+        ("runs_tiles", runs_column(3 * 16384 + 77, 40, 500, 6), 0),
+        ("rid5", runs_column(50000, 20, 300, 7), 5),
+        ("rid30", runs_column(50000, 20, 300, 8), 30),
+        ("rid31", runs_column(50000, 20, 300, 9), 31),
+        ("rid_big", runs_column(70000, 50, 300, 10), 1_000_003),
+        ("zipf_runs", ia.generate(1 << 20, "zipf-runs", seed=3, cardinality=1 << 20, zipf_s=1.2,
+                                  run_mean=4096), 0),
+    ]
+    return out
+
+
+def run_starts(values, rid_base):
+    """Rows that do not repeat their predecessor inside a 31-row chunk."""
+    rows = rid_base + np.arange(values.size, dtype=np.uint64)
+    start = np.ones(values.size, bool)
+    start[1:] = (values[1:] != values[:-1]) | (rows[1:] % 31 == 0)
+    return int(start.sum())
+
+
+@pytest.mark.parametrize("name,values,rid_base", cases(), ids=[c[0] for c in cases()])
+def test_run_mode_matches_oracle(ctx, name, values, rid_base):
+    print(f"[runs] {name} n={values.size} rid_base={rid_base}", flush=True)
+    got = ctx.build(values, rid_base=rid_base)
+    expect_run = 3 * run_starts(values, rid_base) < 2 * values.size
+    assert got.mode == (1 if expect_run else 0)
+    assert_same(got, oracle.build(values, rid_base=rid_base), name)
+
+
+def test_most_cases_use_run_mode():
+    assert sum(3 * run_starts(v, r) < 2 * v.size for _, v, r in cases()) >= len(cases()) - 2
+
+
+def test_uniform_stays_in_pair_mode(ctx):
+    v = np.random.default_rng(1).integers(0, 65536, 100000).astype(np.uint32)
+    got = ctx.build(v)
+    assert got.mode == 0
+    assert_same(got, oracle.build(v))
+
+
+def test_batch_mixes_modes(ctx):
+    rng = np.random.default_rng(12)
+    cols = []
+    for i in range(24):
+        if i % 3 == 0:
+            cols.append(rng.integers(0, 1 << 16, int(rng.integers(1, 40000))).astype(np.uint32))
+        else:
+            cols.append(runs_column(int(rng.integers(1, 60000)), float(rng.uniform(2, 300)),
+                                    int(rng.integers(1, 5000)), 100 + i))
+    buf, layout = pack(cols, gap=1)
+    rid_bases = [(37 * i) % 97 for i in range(len(cols))]
+    spec = [(o, n, r) for (o, n), r in zip(layout, rid_bases)]
+    print(f"[runs] batch of {len(cols)}", flush=True)
+    out = ctx.build_batch_device(dev(buf), spec)
+    counts = ctx.read_counts_batch(out[4], len(cols))
+    got = per_column(out, counts, layout)
+    assert {c.mode for c in counts} == {0, 1}
+    for i, c in enumerate(cols):
+        assert_same(got[i], oracle.build(c, rid_base=rid_bases[i]), f"column {i}")

@@ -126,7 +126,8 @@ struct BuildArgs {
   uint32_t* sort_status;             // 2 arrays of sort_status_stride words
   uint64_t sort_status_stride;
   uint64_t* tail_status;             // tail_tiles_cap words
-  uint64_t* cut_status;              // sort_tiles_cap words (run cut look-back)
+  uint32_t* tile_runs;               // sort_tiles_cap words: runs per radix tile, then
+                                     // their exclusive prefix within the column
   uint32_t *out_keys, *out_offsets, *out_lengths, *out_words;
   uint32_t *sorted_keys, *sorted_rids;  // stage API sort output (one column)
   wahgpu_counts_dev* counts;         // per column

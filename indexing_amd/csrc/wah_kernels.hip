@@ -34,10 +34,12 @@ __global__ __launch_bounds__(kBlock) void k_fill32(uint32_t* __restrict__ p, uin
 // build also gets its descriptor written here (no host-to-device copy).
 __global__ __launch_bounds__(kBlock) void k_begin(Header* __restrict__ hdr,
                                                   ColState* __restrict__ cs, uint32_t cols_cap,
-                                                  DescHead* __restrict__ desc, int inline_desc,
-                                                  Dims one_dims, Column one) {
+                                                  uint32_t* __restrict__ tile_runs,
+                                                  uint32_t tiles_cap, DescHead* __restrict__ desc,
+                                                  int inline_desc, Dims one_dims, Column one) {
   const uint64_t gt = (uint64_t)blockIdx.x * kBlock + threadIdx.x;
   const uint64_t gs = (uint64_t)gridDim.x * kBlock;
+  for (uint64_t i = gt; i < tiles_cap; i += gs) tile_runs[i] = 0;
   uint32_t* h = reinterpret_cast<uint32_t*>(hdr);
   for (uint64_t i = gt; i < sizeof(Header) / 4; i += gs) h[i] = 0;
   const uint64_t zw = (uint64_t)cols_cap * kColZeroWords;
@@ -121,10 +123,11 @@ __device__ __forceinline__ void hist_flush(uint32_t* h, uint32_t* hr, uint32_t* 
 __global__ __launch_bounds__(kBlock) void k_histogram(
     const uint32_t* __restrict__ values, const DescHead* __restrict__ desc,
     ColState* __restrict__ cs, uint32_t* __restrict__ status0,
-    uint64_t* __restrict__ tail_status, uint64_t* __restrict__ cut_status) {
+    uint64_t* __restrict__ tail_status, uint32_t* __restrict__ tile_runs) {
   __shared__ uint32_t h[4 * 256];
   __shared__ uint32_t hr[4 * 256];
   __shared__ uint32_t s_max[kWaves];
+  __shared__ uint32_t s_reps[kWaves];
   const int t = threadIdx.x;
   const Dims dims = desc->dims;
   const Column* cols = desc_cols(desc);
@@ -135,7 +138,6 @@ __global__ __launch_bounds__(kBlock) void k_histogram(
   const uint64_t s0 = (uint64_t)dims.sort_tiles * 256;
   for (uint64_t i = gt; i < s0; i += gthreads) status0[i] = 0;
   for (uint64_t i = gt; i < dims.tail_tiles; i += gthreads) tail_status[i] = 0;
-  for (uint64_t i = gt; i < dims.sort_tiles; i += gthreads) cut_status[i] = 0;
   __syncthreads();
 
   const uint32_t per = (dims.hist_chunks + gridDim.x - 1) / gridDim.x;
@@ -158,6 +160,7 @@ __global__ __launch_bounds__(kBlock) void k_histogram(
     // row of src[i] is rid_base + lo + i; a row repeats its predecessor when
     // it is not the first of the column or of a chunk and has its value
     const uint32_t row0 = cd.rid_base + (uint32_t)lo;
+    uint32_t reps = 0;  // rows of this chunk that repeat their predecessor
     if (cnt == kHistChunk && ((uintptr_t)src % 16) == 0) {
       // four 16-byte loads per thread
       const uint4* v4 = reinterpret_cast<const uint4*>(src);
@@ -176,6 +179,7 @@ __global__ __launch_bounds__(kBlock) void k_histogram(
           vmax = e[j] > vmax ? e[j] : vmax;
           const uint32_t pv = j == 0 ? prev : e[j - 1];
           rep[j] = lo + i0 + j > 0 && (row0 + i0 + j) % 31u != 0u && e[j] == pv;
+          reps += rep[j] ? 1u : 0u;
         }
         // one value in all 256 rows of the wave (runs): one add per bin
         const uint32_t f = __builtin_amdgcn_readfirstlane(e[0]);
@@ -210,9 +214,22 @@ __global__ __launch_bounds__(kBlock) void k_histogram(
         const bool has_prev = valid && lo + i > 0;
         const uint32_t pv = has_prev ? src[(int64_t)i - 1] : 0u;
         vmax = valid && v > vmax ? v : vmax;
-        hist_value(h, hr, v, valid, has_prev && (row0 + i) % 31u != 0u && v == pv);
+        const bool rep = has_prev && (row0 + i) % 31u != 0u && v == pv;
+        reps += rep ? 1u : 0u;
+        hist_value(h, hr, v, valid, rep);
       }
     }
+    // runs of the chunk, added to its radix tile (a run-mode column's cut
+    // places its items from these counts)
+    reps = (uint32_t)wave_sum64(reps);
+    if ((t % kWave) == 0) s_reps[t / kWave] = reps;
+    __syncthreads();
+    if (t == 0) {
+      uint32_t r = 0;
+      for (int i = 0; i < kWaves; ++i) r += s_reps[i];
+      atomicAdd(&tile_runs[cd.sort_tile0 + (uint32_t)(lo / kSortTile)], cnt - r);
+    }
+    __syncthreads();
   }
   hist_flush(h, hr, s_max, vmax, &cs[col]);
 }
@@ -223,6 +240,7 @@ __global__ __launch_bounds__(kBlock) void k_histogram(
 // each one writes back and invalidates the L2).
 __global__ __launch_bounds__(kBlock) void k_plan(const DescHead* __restrict__ desc,
                                                  ColState* __restrict__ cs, Header* __restrict__ hdr,
+                                                 uint32_t* __restrict__ tile_runs,
                                                  wahgpu_counts_dev* __restrict__ counts,
                                                  int force_pair) {
   __shared__ uint64_t s_scan[kWaves];
@@ -243,6 +261,20 @@ __global__ __launch_bounds__(kBlock) void k_plan(const DescHead* __restrict__ de
     const uint64_t runs = n - s_reps;
     const bool run_mode = !force_pair && n > 0 && 3 * runs < 2 * n;
     const uint64_t items = run_mode ? runs : n;
+    if (run_mode) {
+      // where each radix tile's runs go: exclusive prefix of the runs per tile
+      const Column cd = desc_cols(desc)[c];
+      uint32_t* tr = tile_runs + cd.sort_tile0;
+      const uint32_t tiles = sort_tiles_of(n);
+      uint64_t carry = 0;
+      for (uint32_t b = 0; b < tiles; b += kBlock) {
+        const uint32_t x = b + t < tiles ? tr[b + t] : 0u;
+        uint64_t tot;
+        const uint64_t ex = block_exclusive<OpAdd>(x, s_scan, &tot);
+        if (b + t < tiles) tr[b + t] = (uint32_t)(carry + ex);
+        carry += tot;
+      }
+    }
     for (int p = 0; p < 4; ++p) {
       const uint32_t h = col->hist[p][t] - (run_mode ? col->rep[p][t] : 0u);
       uint64_t total;
@@ -296,19 +328,19 @@ __device__ __forceinline__ uint32_t run_length(uint64_t stop, uint32_t l, uint32
 // (a row starts one unless it repeats its predecessor's value inside the
 // same chunk) and writes one item per run, in row order, to buffer B:
 // value, rid of its first row, literal bits of its rows.  An ordered stream
-// compaction: per-wave ballots, a block scan and a look-back over one count
-// per tile; tiles are the radix tiles of the column.
+// compaction without look-back: the histogram kernel counted the runs of
+// each radix tile and k_plan turned the counts into per-tile offsets; per
+// tile, ballots and a block scan over the waves place each run.
 template <int THREADS, int ITEMS>
 __global__ __launch_bounds__(THREADS) void k_runcut(
     const uint32_t* __restrict__ values, uint32_t* __restrict__ keysB,
     uint32_t* __restrict__ ridsB, uint32_t* __restrict__ litsB,
     const DescHead* __restrict__ desc, const ColState* __restrict__ cs,
-    Header* __restrict__ hdr, uint64_t* cut_status, uint32_t lanes) {
+    Header* __restrict__ hdr, const uint32_t* __restrict__ tile_runs, uint32_t lanes) {
   constexpr int TILE = THREADS * ITEMS;
   constexpr int WAVES = THREADS / kWave;
   __shared__ uint32_t s_lead[WAVES + 1];  // leading non-stop lanes of each wave's first 64 rows
   __shared__ uint32_t s_wsum[WAVES];
-  __shared__ uint64_t s_prefix;
   __shared__ uint32_t s_tile;
   const Dims dims = desc->dims;
   const Column* cols = desc_cols(desc);
@@ -377,12 +409,8 @@ __global__ __launch_bounds__(THREADS) void k_runcut(
       before += i < w ? s_wsum[i] : 0u;
       total += s_wsum[i];
     }
-    if (w == 0) {  // wave-parallel look-back: 64 predecessors per round trip
-      const uint64_t pre = lookback_wave_sum(cut_status, tile, cd.sort_tile0, total, &hdr->fault);
-      if (l == 0) s_prefix = pre;
-    }
-    __syncthreads();
-    uint64_t idx = s_prefix + before;
+    (void)total;
+    uint64_t idx = (uint64_t)tile_runs[tile] + before;  // runs of the column's earlier tiles
     const uint64_t items = col->items;
 #pragma unroll
     for (int i = 0; i < ITEMS; ++i) {
@@ -1242,21 +1270,21 @@ static uint32_t cap_grid(uint32_t work, uint32_t cap) {
 }
 
 void launch_sort(const BuildArgs& a, hipStream_t s, LaunchHook* hook) {
-  hipLaunchKernelGGL(k_begin, dim3(64), dim3(kBlock), 0, s, a.hdr, a.cs, a.cols_cap, a.desc,
-                     (int)a.inline_desc, a.one_dims, a.one);
+  hipLaunchKernelGGL(k_begin, dim3(64), dim3(kBlock), 0, s, a.hdr, a.cs, a.cols_cap,
+                     a.tile_runs, a.sort_tiles_cap, a.desc, (int)a.inline_desc, a.one_dims, a.one);
   uint64_t hb = (a.n_cap + 4 * kHistChunk - 1) / (4 * kHistChunk);
   if (hb > a.hist_chunks_cap) hb = a.hist_chunks_cap;
   hipLaunchKernelGGL(k_histogram, dim3(cap_grid((uint32_t)hb, kHistMaxBlocks)), dim3(kBlock), 0,
-                     s, a.values, a.desc, a.cs, a.sort_status, a.tail_status, a.cut_status);
+                     s, a.values, a.desc, a.cs, a.sort_status, a.tail_status, a.tile_runs);
   hipLaunchKernelGGL(k_plan, dim3(cap_grid(a.cols_cap, 1024)), dim3(kBlock), 0, s, a.desc, a.cs,
-                     a.hdr, a.counts, (int)a.force_pair);
+                     a.hdr, a.tile_runs, a.counts, (int)a.force_pair);
   const uint32_t grid =
       cap_grid(a.sort_tiles_cap, WAH_STATIC_TILES ? a.sort_tiles_cap
                                                   : a.max_resident * 256 / kSortThreads);
   // the cut needs half the registers of a radix pass: two blocks per CU
   const uint32_t cut_grid = cap_grid(a.sort_tiles_cap, 2 * grid);
   hipLaunchKernelGGL((k_runcut<kSortThreads, kSortItems>), dim3(cut_grid), dim3(kSortThreads), 0,
-                     s, a.values, a.keysB, a.ridsB, a.litsB, a.desc, a.cs, a.hdr, a.cut_status,
+                     s, a.values, a.keysB, a.ridsB, a.litsB, a.desc, a.cs, a.hdr, a.tile_runs,
                      tile_lanes(cut_grid));
   if (hook) hook->mark(s);
   // each launch runs the a-th pass of pair-mode columns, then of run-mode

@@ -248,6 +248,7 @@ def main():
     elapsed = time.perf_counter() - t0
     group_ms, calls = ctx.timing_end()
     counts = ctx.read_counts(out[4])
+    rank_mode = ctx.rank_mode
     assert counts.fault == 0, "device reported a fault"
     assert calls == args.steps
 
@@ -287,7 +288,8 @@ def main():
         "config": {"workload": f"single-column WAH build, n=2^{n.bit_length() - 1} uint32 "
                                f"{column}, resident in HBM",
                    "n": n, "k": int(counts.k), "L": L, "K": K, "radix_passes": P,
-                   "sort_mode": "runs" if counts.mode else "pairs"},
+                   "sort_mode": "runs" if counts.mode else "pairs",
+                   "radix_ranking": "lds-fetch-add" if rank_mode else "ballot-match"},
         "roofline": {"bound": "hbm", "kernel": "k_onesweep (radix pass, mean of active launches)",
                      "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4),

@@ -89,7 +89,12 @@ void* wahgpu_stream(wahgpu_ctx* ctx);               /* the hipStream_t */
  * stream that is ordered with the legacy default stream (torch's default). */
 int wahgpu_set_stream(wahgpu_ctx* ctx, void* stream);
 int wahgpu_synchronize(wahgpu_ctx* ctx);
-uint64_t wahgpu_arena_bytes(const wahgpu_ctx* ctx); /* currently reserved */
+uint64_t wahgpu_arena_bytes(const wahgpu_ctx* ctx);
+/* Radix ranking the context uses: 1 = LDS fetch-add (taken when a check at
+ * wahgpu_create shows the device returns same-address fetch-adds of one wave
+ * instruction in lane order, which makes the rank stable), 0 = ballot match.
+ * The environment variable WAHGPU_RANK=match forces 0. */
+int wahgpu_rank_mode(const wahgpu_ctx* ctx); /* currently reserved */
 
 /* ---- whole build (the indexer's worker, src/gpu.cpp:356-367; the phases
  * pipeline, src/phases.cpp:720-1088) ---------------------------------------

@@ -130,6 +130,11 @@ class Context:
     def arena_bytes(self):
         return int(self._lib.wahgpu_arena_bytes(self._h))
 
+    @property
+    def rank_mode(self):
+        """1: radix ranking by LDS fetch-add (device-checked), 0: ballot match."""
+        return int(self._lib.wahgpu_rank_mode(self._h))
+
     def _empty(self, n):
         torch = _torch()
         return torch.empty(max(int(n), 0), dtype=torch.int32, device=f"cuda:{self.device}")

@@ -81,6 +81,7 @@ SIGNATURES = {
     "wahgpu_set_stream": (ctypes.c_int, [vp, vp]),
     "wahgpu_synchronize": (ctypes.c_int, [vp]),
     "wahgpu_arena_bytes": (ctypes.c_uint64, [vp]),
+    "wahgpu_rank_mode": (ctypes.c_int, [vp]),
     "wahgpu_build_device": (ctypes.c_int, [vp, vp, ctypes.c_uint64, ctypes.c_uint32, vp, vp,
                                            vp, vp, vp]),
     "wahgpu_timing_begin": (ctypes.c_int, [vp, ctypes.c_uint32]),

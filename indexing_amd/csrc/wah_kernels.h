@@ -138,6 +138,8 @@ struct BuildArgs {
   uint64_t n_cap;
   // Every column sorts (value, rid) pairs (the stage API needs them).
   bool force_pair;
+  // Radix ranking by LDS fetch-add (checked on the device at context start).
+  bool atomic_rank;
   // Single column: k_begin writes the descriptor from these (no host copy).
   bool inline_desc;
   Dims one_dims;
@@ -173,6 +175,8 @@ struct LaunchHook {
 void launch_sort(const BuildArgs& a, hipStream_t s, LaunchHook* hook = nullptr);
 void launch_tail(const BuildArgs& a, hipStream_t s, LaunchHook* hook = nullptr);
 void launch_sort_finalize(const BuildArgs& a, hipStream_t s);
+// *ok must hold 1; stays 1 if LDS fetch-adds rank lanes in lane order here.
+void launch_rank_selftest(uint32_t* ok, hipStream_t s);
 void launch_fill32(uint32_t* p, uint64_t words, uint32_t v, hipStream_t s);
 void launch_chunks(const uint32_t* rids, uint64_t n, uint32_t* chids, uint32_t* lits,
                    hipStream_t s);

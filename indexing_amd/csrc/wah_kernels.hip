@@ -434,7 +434,51 @@ __global__ __launch_bounds__(THREADS) void k_runcut(
   }
 }
 
-template <int THREADS, int ITEMS, bool RUN>
+// Ranking of a radix pass.  kRankMatch: peers = lanes with the same digit
+// from 8 ballots, rank = peers below + earlier count (stable by
+// construction, ~50 VALU per key).  kRankAtomic: one LDS fetch-add per key on
+// the wave's digit counter; the returned counts are a stable rank because the
+// LDS serializes the lanes of one instruction that hit one address in lane
+// order -- not a documented guarantee, so a context takes this path only
+// after k_rank_selftest has checked it on the device.
+constexpr int kRankMatch = 0;
+constexpr int kRankAtomic = 1;
+
+// Checks on this device that LDS fetch-adds of one wave instruction return,
+// for lanes on the same address, counts in increasing lane order: many
+// address patterns (1 to 64 lanes per address, every lane order), repeated.
+// *ok = 1 if every check held.
+__global__ __launch_bounds__(256) void k_rank_selftest(uint32_t* ok) {
+  __shared__ uint32_t cnt[4][256];
+  const int t = threadIdx.x, w = t / kWave, l = t % kWave;
+  uint32_t good = 1;
+  for (uint32_t round = 0; round < 512; ++round) {
+    for (int i = t; i < 4 * 256; i += 256) (&cnt[0][0])[i] = 0;
+    __syncthreads();
+    // address pattern of this round: few to many distinct digits
+    const uint32_t h = (uint32_t)l * 2654435761u ^ (round * 40503u + (uint32_t)w * 977u);
+    const uint32_t span = 1u + (round % 64u);
+    const uint32_t d = (h >> 7) % span;
+    uint32_t prev = 0;
+    for (int rep = 0; rep < 4; ++rep) {
+      const uint32_t r = atomicAdd(&cnt[w][d], 1u);
+      uint64_t peers = ballot(true);
+#pragma unroll
+      for (int b = 0; b < 8; ++b) {
+        const bool bit = (d >> b) & 1u;
+        const uint64_t bb = ballot(bit);
+        peers &= bit ? bb : ~bb;
+      }
+      const uint32_t expect = (uint32_t)rep * (uint32_t)__popcll(peers) + mbcnt(peers);
+      if (r != expect || (rep > 0 && r <= prev)) good = 0;
+      prev = r;
+    }
+    __syncthreads();
+  }
+  if (!good) atomicAnd(ok, 0u);
+}
+
+template <int THREADS, int ITEMS, bool RUN, int RANK>
 __global__ __launch_bounds__(THREADS, WAH_SORT_WAVES_PER_EU) void k_onesweep(
     uint32_t a, const uint32_t* __restrict__ values, uint32_t* keysA, uint32_t* ridsA,
     uint32_t* keysB, uint32_t* ridsB, uint32_t* litsA, uint32_t* litsB,
@@ -520,18 +564,22 @@ __global__ __launch_bounds__(THREADS, WAH_SORT_WAVES_PER_EU) void k_onesweep(
     for (int i = 0; i < ITEMS; ++i) {
       if ((part >> i) & 1u) {
         const uint32_t d = (key[i] >> shift) & 255u;
-        uint64_t peers = ballot(true);
+        if (RANK == kRankAtomic) {
+          rank[i] = atomicAdd(&s_wcnt[w][d], 1u);
+        } else {
+          uint64_t peers = ballot(true);
 #pragma unroll
-        for (int b = 0; b < 8; ++b) {
-          const bool bit = (d >> b) & 1u;
-          const uint64_t bb = ballot(bit);
-          peers &= bit ? bb : ~bb;
+          for (int b = 0; b < 8; ++b) {
+            const bool bit = (d >> b) & 1u;
+            const uint64_t bb = ballot(bit);
+            peers &= bit ? bb : ~bb;
+          }
+          const uint32_t cnt = (uint32_t)__popcll(peers);
+          const uint32_t below = mbcnt(peers);
+          const uint32_t prior = s_wcnt[w][d];
+          rank[i] = prior + below;
+          if (below == cnt - 1) s_wcnt[w][d] = prior + cnt;
         }
-        const uint32_t cnt = (uint32_t)__popcll(peers);
-        const uint32_t below = mbcnt(peers);
-        const uint32_t prior = s_wcnt[w][d];
-        rank[i] = prior + below;
-        if (below == cnt - 1) s_wcnt[w][d] = prior + cnt;
       }
     }
     __syncthreads();
@@ -1290,14 +1338,25 @@ void launch_sort(const BuildArgs& a, hipStream_t s, LaunchHook* hook) {
   // each launch runs the a-th pass of pair-mode columns, then of run-mode
   // columns; a kernel with no such column returns at once
   for (uint32_t p = 0; p < 4; ++p) {
-    hipLaunchKernelGGL((k_onesweep<kSortThreads, kSortItems, false>), dim3(grid),
-                       dim3(kSortThreads), 0, s, p, a.values, a.keysA, a.ridsA, a.keysB, a.ridsB,
-                       a.litsA, a.litsB, a.desc, a.cs, a.hdr, a.sort_status, a.sort_status_stride,
-                       tile_lanes(grid));
-    hipLaunchKernelGGL((k_onesweep<kSortThreads, kSortItems, true>), dim3(grid),
-                       dim3(kSortThreads), 0, s, p, a.values, a.keysA, a.ridsA, a.keysB, a.ridsB,
-                       a.litsA, a.litsB, a.desc, a.cs, a.hdr, a.sort_status, a.sort_status_stride,
-                       tile_lanes(grid));
+    if (a.atomic_rank) {
+      hipLaunchKernelGGL((k_onesweep<kSortThreads, kSortItems, false, kRankAtomic>), dim3(grid),
+                         dim3(kSortThreads), 0, s, p, a.values, a.keysA, a.ridsA, a.keysB,
+                         a.ridsB, a.litsA, a.litsB, a.desc, a.cs, a.hdr, a.sort_status,
+                         a.sort_status_stride, tile_lanes(grid));
+      hipLaunchKernelGGL((k_onesweep<kSortThreads, kSortItems, true, kRankAtomic>), dim3(grid),
+                         dim3(kSortThreads), 0, s, p, a.values, a.keysA, a.ridsA, a.keysB,
+                         a.ridsB, a.litsA, a.litsB, a.desc, a.cs, a.hdr, a.sort_status,
+                         a.sort_status_stride, tile_lanes(grid));
+    } else {
+      hipLaunchKernelGGL((k_onesweep<kSortThreads, kSortItems, false, kRankMatch>), dim3(grid),
+                         dim3(kSortThreads), 0, s, p, a.values, a.keysA, a.ridsA, a.keysB,
+                         a.ridsB, a.litsA, a.litsB, a.desc, a.cs, a.hdr, a.sort_status,
+                         a.sort_status_stride, tile_lanes(grid));
+      hipLaunchKernelGGL((k_onesweep<kSortThreads, kSortItems, true, kRankMatch>), dim3(grid),
+                         dim3(kSortThreads), 0, s, p, a.values, a.keysA, a.ridsA, a.keysB,
+                         a.ridsB, a.litsA, a.litsB, a.desc, a.cs, a.hdr, a.sort_status,
+                         a.sort_status_stride, tile_lanes(grid));
+    }
     if (hook) hook->mark(s);
   }
 }
@@ -1326,6 +1385,10 @@ void launch_sort_finalize(const BuildArgs& a, hipStream_t s) {
   if (g == 0) return;
   hipLaunchKernelGGL(k_sort_finalize, dim3((uint32_t)g), dim3(kBlock), 0, s, a.desc, a.cs,
                      a.values, a.keysA, a.ridsA, a.keysB, a.ridsB, a.sorted_keys, a.sorted_rids);
+}
+
+void launch_rank_selftest(uint32_t* ok, hipStream_t s) {
+  hipLaunchKernelGGL(k_rank_selftest, dim3(64), dim3(256), 0, s, ok);
 }
 
 uint64_t scan_status_words(uint64_t n) { return (n + kScanTile - 1) / kScanTile; }

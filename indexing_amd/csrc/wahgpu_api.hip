@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 #include <stdio.h>
+#include <stdlib.h>
 #include <string.h>
 
 #include <chrono>
@@ -41,6 +42,7 @@ struct wahgpu_ctx {
   hipGraphExec_t gexec = nullptr;
   std::vector<uintptr_t> gkey;  // buffers and sizes the graph was captured with
   uint64_t captures = 0, replays = 0;
+  bool atomic_rank = false;  // radix ranking by LDS fetch-add (self-tested)
 };
 
 namespace {
@@ -146,6 +148,7 @@ void carve_build(wahgpu_ctx* c, BuildArgs* a, const Caps& k) {
   a->tail_status = take<uint64_t>(c, (uint64_t)k.tail_tiles + 1);
   a->tile_runs = take<uint32_t>(c, (uint64_t)k.sort_tiles + 1);
   a->max_resident = c->max_resident;
+  a->atomic_rank = c->atomic_rank;
   a->n_cap = k.n;
   a->cols_cap = k.cols;
   a->sort_tiles_cap = k.sort_tiles;
@@ -314,6 +317,29 @@ int wahgpu_create(wahgpu_ctx** out, int device_ordinal, uint64_t arena_bytes) {
   hipDeviceProp_t prop;
   if (hipGetDeviceProperties(&prop, device_ordinal) == hipSuccess)
     c->max_resident = (uint32_t)prop.multiProcessorCount * 4u;
+  // Radix ranking: LDS fetch-add when the device ranks same-address lanes in
+  // lane order (checked here), else the ballot match.  WAHGPU_RANK=match or
+  // =atomic overrides (the latter still requires the check to pass).
+  {
+    const char* env = getenv("WAHGPU_RANK");
+    const std::string want = env ? env : "auto";
+    uint32_t* d_ok = nullptr;
+    if (want != "match" && hipMalloc(&d_ok, 4) == hipSuccess) {
+      *c->pinned_counts = wahgpu_counts{};
+      c->pinned_counts->bound = 1;
+      uint32_t one = 1;
+      if (hipMemcpyAsync(d_ok, &c->pinned_counts->bound, 4, hipMemcpyHostToDevice,
+                         c->stream) == hipSuccess) {
+        wah::launch_rank_selftest(d_ok, c->stream);
+        if (hipMemcpyAsync(&c->pinned_counts->bound, d_ok, 4, hipMemcpyDeviceToHost,
+                           c->stream) == hipSuccess &&
+            hipStreamSynchronize(c->stream) == hipSuccess)
+          c->atomic_rank = c->pinned_counts->bound == one;
+      }
+      (void)hipFree(d_ok);
+    }
+    (void)hipGetLastError();
+  }
   if (arena_bytes) {
     c->fixed = false;
     int rc = arena_reserve(c, arena_bytes);
@@ -350,6 +376,7 @@ int wahgpu_set_stream(wahgpu_ctx* c, void* stream) {
   return WAHGPU_OK;
 }
 uint64_t wahgpu_arena_bytes(const wahgpu_ctx* c) { return c ? c->arena_cap : 0; }
+int wahgpu_rank_mode(const wahgpu_ctx* c) { return c && c->atomic_rank ? 1 : 0; }
 
 int wahgpu_synchronize(wahgpu_ctx* c) {
   if (!c) return WAHGPU_EINVAL;

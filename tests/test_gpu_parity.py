@@ -187,3 +187,22 @@ def test_primitives_match_oracle(ctx, n):
     torch.cuda.synchronize()
     assert np.array_equal(u32(ex), oracle.exclusive_scan(data))
     assert np.array_equal(u32(seg), oracle.segmented_inclusive_scan(data, heads))
+
+
+def test_both_rankings_match_oracle(ctx, monkeypatch):
+    """The LDS fetch-add ranking (taken after the device check) and the
+    ballot-match ranking give the same index; columns with 1-4 distinct
+    digits put many lanes on one counter."""
+    monkeypatch.setenv("WAHGPU_RANK", "match")
+    m = ia.Context(0)
+    try:
+        assert m.rank_mode == 0
+        print(f"[rank] default context rank mode {ctx.rank_mode}", flush=True)
+        rng = np.random.default_rng(77)
+        for card in (2, 3, 4, 256, 65536):
+            v = (rng.integers(0, card, 300001) * 65793).astype(np.uint32)
+            ref = oracle.build(v)
+            assert_same(ctx.build(v), ref, f"default card={card}")
+            assert_same(m.build(v), ref, f"match card={card}")
+    finally:
+        m.close()

@@ -174,34 +174,35 @@ __global__ __launch_bounds__(kBlock) void k_histogram(
         if (lane_id() == 0) prev = (lo + i0 > 0) ? src[(int64_t)i0 - 1] : 0u;
         const uint32_t e[4] = {q[u].x, q[u].y, q[u].z, q[u].w};
         bool rep[4];
+        uint32_t m = (row0 + i0) % 31u;  // row % 31 of e[0]
 #pragma unroll
         for (int j = 0; j < 4; ++j) {
           vmax = e[j] > vmax ? e[j] : vmax;
           const uint32_t pv = j == 0 ? prev : e[j - 1];
-          rep[j] = lo + i0 + j > 0 && (row0 + i0 + j) % 31u != 0u && e[j] == pv;
+          rep[j] = lo + i0 + j > 0 && m != 0u && e[j] == pv;
           reps += rep[j] ? 1u : 0u;
+          m = m == 30u ? 0u : m + 1u;
         }
-        // one value in all 256 rows of the wave (runs): one add per bin
+        // Digits that are the same in all 256 rows of the wave (the high
+        // digits of small values, every digit of a run) take one add per
+        // wave; the others one LDS atomic per row.
         const uint32_t f = __builtin_amdgcn_readfirstlane(e[0]);
-        if (ballot(e[0] != f || e[1] != f || e[2] != f || e[3] != f) == 0) {
-          uint32_t rc = 0;
+        const uint32_t vary = wave_or((e[0] ^ f) | (e[1] ^ f) | (e[2] ^ f) | (e[3] ^ f));
+        uint32_t rc = 0;
 #pragma unroll
-          for (int j = 0; j < 4; ++j) rc += (uint32_t)__popcll(ballot(rep[j]));
-          if (lane_id() == 0) {
+        for (int j = 0; j < 4; ++j) rc += (uint32_t)__popcll(ballot(rep[j]));
 #pragma unroll
-            for (int p = 0; p < 4; ++p) {
+        for (int p = 0; p < 4; ++p) {
+          if (((vary >> (8 * p)) & 255u) == 0u) {
+            if (lane_id() == 0) {
               atomicAdd(&h[p * 256 + ((f >> (8 * p)) & 255u)], 4u * kWave);
               if (rc) atomicAdd(&hr[p * 256 + ((f >> (8 * p)) & 255u)], rc);
             }
-          }
-        } else {
+          } else {
 #pragma unroll
-          for (int j = 0; j < 4; ++j) {
-#pragma unroll
-            for (int p = 0; p < 4; ++p) hist_add(h, p * 256 + ((e[j] >> (8 * p)) & 255u), true);
-            if (ballot(rep[j])) {
-#pragma unroll
-              for (int p = 0; p < 4; ++p) hist_add(hr, p * 256 + ((e[j] >> (8 * p)) & 255u), rep[j]);
+            for (int j = 0; j < 4; ++j) {
+              atomicAdd(&h[p * 256 + ((e[j] >> (8 * p)) & 255u)], 1u);
+              if (rc && rep[j]) atomicAdd(&hr[p * 256 + ((e[j] >> (8 * p)) & 255u)], 1u);
             }
           }
         }
@@ -370,19 +371,23 @@ __global__ __launch_bounds__(THREADS) void k_runcut(
       key[i] = pos < n ? vp[pos] : 0u;
     }
     uint32_t mine = 0;  // run starts of this wave
+    uint32_t m = (cd.rid_base + (uint32_t)wbase) % 31u;  // row % 31 of item i (rows 64 apart)
 #pragma unroll
     for (int i = 0; i < ITEMS; ++i) {
       const uint64_t pos = wbase + (uint64_t)(i * kWave);
       const bool valid = pos < n;
-      uint32_t prev = (uint32_t)__shfl_up((int)key[i], 1, kWave);
+      // previous row: lane l - 1 (wave shift by one lane), lane 0 of item i
+      // from lane 63 of item i - 1, lane 0 of item 0 from memory
+      uint32_t prev = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)key[i], 0x138, 0xF, 0xF, false);
       if (i > 0) {
         const uint32_t last = readlane(key[i > 0 ? i - 1 : 0], kWave - 1);
         if (l == 0) prev = last;
       } else if (l == 0 && valid && pos > 0) {
         prev = vp[pos - 1];
       }
-      const bool start = valid && (pos == 0 || (cd.rid_base + (uint32_t)pos) % 31u == 0u ||
-                                   key[i] != prev);
+      const bool start = valid && (pos == 0 || m == 0u || key[i] != prev);
+      m += 2u;  // 64 rows on: 64 % 31 = 2
+      m = m >= 31u ? m - 31u : m;
       startm[i] = ballot(start);
       stop[i] = ballot(start || !valid);
       mine += (uint32_t)__popcll(startm[i]);
@@ -866,11 +871,19 @@ __global__ __launch_bounds__(THREADS, 4) void k_tail(
       key[i] = 0;
       bit[i] = 0;
       uint32_t p = 0;
+      // the previous item is lane l - 1 (wave shift right by one lane);
+      // lane 0 reads it from LDS
+      const uint32_t v = s_k[s], r = s_r[s], c = r / 31u;
+      uint32_t vprev = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x138, 0xF, 0xF, false);
+      uint32_t cprev = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)c, 0x138, 0xF, 0xF, false);
+      if (l == 0) {
+        vprev = s_k[s - 1];
+        cprev = s_r[s - 1] / 31u;
+      }
       if (g < n) {
-        const uint32_t v = s_k[s], r = s_r[s], c = r / 31u;
         const bool first = g == 0;
-        const uint32_t cp = first ? 0u : s_r[s - 1] / 31u;
-        const bool kh = first || v != s_k[s - 1];
+        const uint32_t cp = first ? 0u : cprev;
+        const bool kh = first || v != vprev;
         bit[i] = RUN ? s_l[s] : 1u << (r % 31u);
         if (kh || c != cp) {
           lit[i] = 0x80000000u;  // entry start; payload bits added in pass 2
@@ -933,9 +946,17 @@ __global__ __launch_bounds__(THREADS, 4) void k_tail(
       uint32_t nlo = s_lead[2 * (w + 1)], nhi = s_lead[2 * (w + 1) + 1];
 #pragma unroll
       for (int i = ITEMS - 1; i >= 0; --i) {
+        const uint64_t hm = ballot(lit[i] != 0u);
+        if (hm == ballot(bit[i] != 0u)) {
+          // every item of these 64 starts its own entry (single rows): the
+          // literal is the item's own bits, plus, for lane 63, the run that
+          // continues into the next 64
+          if (lit[i]) lit[i] = 0x80000000u | bit[i] | (l == kWave - 1 ? (nhi << 16) | nlo : 0u);
+          nlo = nhi = 0;  // lane 0 starts an entry or holds no item
+          continue;
+        }
         const uint32_t blo = bit[i] & 0xFFFFu, bhi = bit[i] >> 16;
         const uint32_t slo = wave_incl_add(blo), shi = wave_incl_add(bhi);
-        const uint64_t hm = ballot(lit[i] != 0u);
         const uint64_t rest = l == kWave - 1 ? 0ull : (hm >> (l + 1));
         const int nh = rest ? l + 1 + __builtin_ctzll(rest) : kWave;
         const uint32_t elo = (uint32_t)__shfl((int)slo, nh - 1, kWave);

@@ -92,6 +92,7 @@ def bench_batched(ncols, seed=7, reps=3):
     d_vals = torch.from_numpy(buf.view(np.int32)).to("cuda:0")
     del buf
     ctx = ia.Context(0)
+    tuning = ctx.autotune()
     ctx.batch_reserve(d_vals.numel(), ncols)
     out = ctx.build_batch_device(d_vals, layout)          # captures the graph
     torch.cuda.synchronize()
@@ -131,7 +132,8 @@ def bench_batched(ncols, seed=7, reps=3):
             "batched_gvalues_s": round(total / best_b / 1e6, 2),
             "per_column_ms": round(best_p, 3),
             "per_column_gvalues_s": round(total / best_p / 1e6, 2),
-            "speedup": round(best_p / best_b, 2), "graph_captures": captures,
+            "speedup": round(best_p / best_b, 2), "tile_shapes": tuning["shape"],
+            "graph_captures": captures,
             "graph_replays": replays, "host_generation_s": round(gen_s, 1)}
 
 
@@ -234,6 +236,7 @@ def main():
         column = "Zipf(1.2) over 2^20 values in runs of mean length 4096, seed 1"
     d_values = torch.from_numpy(host.view(np.int32)).to("cuda:0")
     ctx = ia.Context(0)
+    tuning = ctx.autotune()                            # tile shape per length bucket
     out = ctx.build_device(d_values)                   # allocates outputs + arena
     for _ in range(args.warmup):
         ctx.build_device(d_values, out=out)
@@ -262,8 +265,8 @@ def main():
     active = sorted(sorted(range(4), key=lambda p: -pass_ms[p])[:P])
     if counts.mode == 0:
         pass_bytes = [12.0 * n] + [16.0 * n] * (P - 1)     # per active launch, in order
-    else:  # run mode: launch 0 reads the rows, every launch moves 12-byte run items (>= k)
-        pass_bytes = [4.0 * n + 12.0 * counts.k] + [24.0 * counts.k] * (P - 1)
+    else:  # run mode: every launch moves 12-byte run items (at least k of them) in and out
+        pass_bytes = [24.0 * counts.k] * P
     sort_ms = sum(pass_ms[p] for p in active)
     per_launch_bytes = sum(pass_bytes) / P
     per_launch_ms = sort_ms / P
@@ -289,7 +292,8 @@ def main():
                                f"{column}, resident in HBM",
                    "n": n, "k": int(counts.k), "L": L, "K": K, "radix_passes": P,
                    "sort_mode": "runs" if counts.mode else "pairs",
-                   "radix_ranking": "lds-fetch-add" if rank_mode else "ballot-match"},
+                   "radix_ranking": "lds-fetch-add" if rank_mode else "ballot-match",
+                   "tuning": tuning},
         "roofline": {"bound": "hbm", "kernel": "k_onesweep (radix pass, mean of active launches)",
                      "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4),

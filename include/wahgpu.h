@@ -187,6 +187,29 @@ int wahgpu_batch_reserve(wahgpu_ctx* ctx, uint64_t max_values, uint32_t max_cols
 /* Number of graph captures and replays so far (for tests and tuning). */
 int wahgpu_graph_stats(const wahgpu_ctx* ctx, uint64_t* captures, uint64_t* replays);
 
+/* Tile shapes per column-length bucket.  A build sorts each column with the
+ * radix tile shape of its length bucket (columns of different shapes in one
+ * batch run as one launch set per shape):
+ *   buckets:  n < 2^14, n < 2^17, n < 2^20, larger;
+ *   shapes:   0 = 1024 threads x 16 keys, 1 = 512 x 16, 2 = 256 x 16
+ *             (8-bit digits in every shape; the tail tile is 1024 x 8).
+ * wahgpu_autotune times every shape on synthetic batches of each bucket
+ * (uniform values, 2^12 distinct, about 2^24 values per bucket) on this
+ * device, keeps the fastest per bucket in the context and reports it with
+ * the timings.  The default table is shape 0 everywhere. */
+#define WAHGPU_TUNE_BUCKETS 4
+#define WAHGPU_TUNE_SHAPES 3
+typedef struct wahgpu_tuning {
+  uint32_t bucket_log2[WAHGPU_TUNE_BUCKETS]; /* bucket b: n < 2^bucket_log2[b] (64: any) */
+  uint32_t shape[WAHGPU_TUNE_BUCKETS];       /* chosen shape per bucket */
+  uint32_t digit_bits[WAHGPU_TUNE_BUCKETS];  /* radix digit width per bucket (8) */
+  double ms[WAHGPU_TUNE_BUCKETS][WAHGPU_TUNE_SHAPES]; /* autotune: build time per shape */
+} wahgpu_tuning;
+int wahgpu_autotune(wahgpu_ctx* ctx, wahgpu_tuning* out);
+int wahgpu_get_tuning(const wahgpu_ctx* ctx, wahgpu_tuning* out);
+/* Installs shape[] (other fields ignored); shapes must be < WAHGPU_TUNE_SHAPES. */
+int wahgpu_set_tuning(wahgpu_ctx* ctx, const wahgpu_tuning* in);
+
 /* ---- serving: continuous batching of build requests ----------------------
  * A host request loop in place of the indexer -> worker round trips
  * (src/gpu.cpp:194-261): requests (one column each, host memory) queue as

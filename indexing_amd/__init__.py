@@ -262,6 +262,32 @@ class Context:
         one captured hipGraph ((0, 0) turns it off)."""
         self._check(self._lib.wahgpu_batch_reserve(self._h, int(max_values), int(max_cols)))
 
+    def autotune(self):
+        """Time every radix tile shape per column-length bucket on this device
+        and keep the fastest (wahgpu_autotune); returns tuning()."""
+        t = N.Tuning()
+        self._bind_stream()
+        self._check(self._lib.wahgpu_autotune(self._h, ctypes.byref(t)))
+        return self._tuning_dict(t)
+
+    def tuning(self):
+        t = N.Tuning()
+        self._check(self._lib.wahgpu_get_tuning(self._h, ctypes.byref(t)))
+        return self._tuning_dict(t)
+
+    def set_tuning(self, shapes):
+        """shapes: radix tile shape per bucket (0: 1024x16, 1: 512x16, 2: 256x16)."""
+        t = N.Tuning()
+        for b, sh in enumerate(shapes):
+            t.shape[b] = int(sh)
+        self._check(self._lib.wahgpu_set_tuning(self._h, ctypes.byref(t)))
+
+    @staticmethod
+    def _tuning_dict(t):
+        return {"bucket_log2": list(t.bucket_log2), "shape": list(t.shape),
+                "digit_bits": list(t.digit_bits),
+                "ms": [[round(x, 4) for x in row] for row in t.ms]}
+
     def graph_stats(self):
         cap, rep = ctypes.c_uint64(0), ctypes.c_uint64(0)
         self._check(self._lib.wahgpu_graph_stats(self._h, ctypes.byref(cap), ctypes.byref(rep)))

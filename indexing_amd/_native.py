@@ -68,6 +68,16 @@ class ServerStats(ctypes.Structure):
                 ("max_us", ctypes.c_double), ("busy_s", ctypes.c_double)]
 
 
+TUNE_BUCKETS, TUNE_SHAPES = 4, 3
+
+
+class Tuning(ctypes.Structure):
+    _fields_ = [("bucket_log2", ctypes.c_uint32 * TUNE_BUCKETS),
+                ("shape", ctypes.c_uint32 * TUNE_BUCKETS),
+                ("digit_bits", ctypes.c_uint32 * TUNE_BUCKETS),
+                ("ms", (ctypes.c_double * TUNE_SHAPES) * TUNE_BUCKETS)]
+
+
 # name -> (restype, argtypes); every function the header declares.
 SIGNATURES = {
     "wahgpu_device_count": (ctypes.c_int, [ctypes.POINTER(ctypes.c_int)]),
@@ -101,6 +111,9 @@ SIGNATURES = {
                                           ctypes.c_uint32, ctypes.POINTER(Result)]),
     "wahgpu_batch_reserve": (ctypes.c_int, [vp, ctypes.c_uint64, ctypes.c_uint32]),
     "wahgpu_graph_stats": (ctypes.c_int, [vp, c_u64p, c_u64p]),
+    "wahgpu_autotune": (ctypes.c_int, [vp, ctypes.POINTER(Tuning)]),
+    "wahgpu_get_tuning": (ctypes.c_int, [vp, ctypes.POINTER(Tuning)]),
+    "wahgpu_set_tuning": (ctypes.c_int, [vp, ctypes.POINTER(Tuning)]),
     "wahgpu_server_create": (ctypes.c_int, [ctypes.POINTER(vp), ctypes.POINTER(ServerConfig)]),
     "wahgpu_server_destroy": (None, [vp]),
     "wahgpu_server_submit": (ctypes.c_int, [vp, vp, ctypes.c_uint64, ctypes.c_uint32,

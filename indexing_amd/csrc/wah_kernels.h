@@ -31,6 +31,13 @@ namespace wah {
 constexpr int kSortThreads = WAH_SORT_THREADS;  // threads of a radix-pass block
 constexpr int kSortItems = WAH_SORT_ITEMS;      // keys per thread in a radix pass
 constexpr int kSortTile = kSortThreads * kSortItems;
+// Radix tile shapes compiled in (shape 0 is the default); a build runs one
+// shape, chosen per column-length bucket by the tuning table.
+constexpr int kRadixShapes = 3;
+constexpr int kRadixThreads[kRadixShapes] = {kSortThreads, 512, 256};
+__host__ __device__ inline uint32_t radix_tile(int shape) {
+  return shape == 1 ? 512u * kSortItems : shape == 2 ? 256u * kSortItems : (uint32_t)kSortTile;
+}
 constexpr int kTailThreads = WAH_TAIL_THREADS;
 constexpr int kTailItems = WAH_TAIL_ITEMS;
 constexpr int kTailTile = kTailThreads * kTailItems;  // sorted pairs per tail tile
@@ -65,8 +72,10 @@ struct Column {
   uint32_t sort_tile0;
   uint32_t tail_tile0;
   uint32_t hist_chunk0;
+  uint32_t index;  // slot of the column's counts in the caller's array
+  uint32_t pad[3];
 };
-static_assert(sizeof(Column) == 32, "descriptor layout");
+static_assert(sizeof(Column) == 48, "descriptor layout");
 
 struct Dims {
   uint32_t ncols, sort_tiles, tail_tiles, hist_chunks;
@@ -132,6 +141,7 @@ struct BuildArgs {
   uint32_t *sorted_keys, *sorted_rids;  // stage API sort output (one column)
   wahgpu_counts_dev* counts;         // per column
   uint32_t max_resident;             // resident 256-thread blocks on the device
+  int shape;                         // radix tile shape (kRadixThreads[shape] x kSortItems)
   // Upper bounds the grids are sized for (a captured graph replays any batch
   // within them; the kernels read the actual sizes from the descriptor).
   uint32_t cols_cap, sort_tiles_cap, tail_tiles_cap, hist_chunks_cap;
@@ -147,8 +157,8 @@ struct BuildArgs {
 };
 
 // Tile / chunk counts of a column of n values.
-__host__ __device__ inline uint32_t sort_tiles_of(uint64_t n) {
-  return (uint32_t)((n + kSortTile - 1) / kSortTile);
+__host__ __device__ inline uint32_t sort_tiles_of(uint64_t n, uint32_t tile = kSortTile) {
+  return (uint32_t)((n + tile - 1) / tile);
 }
 __host__ __device__ inline uint32_t tail_tiles_of(uint64_t n) {
   return (uint32_t)((n + kTailTile - 1) / kTailTile);

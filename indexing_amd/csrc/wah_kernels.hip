@@ -123,7 +123,7 @@ __device__ __forceinline__ void hist_flush(uint32_t* h, uint32_t* hr, uint32_t* 
 __global__ __launch_bounds__(kBlock) void k_histogram(
     const uint32_t* __restrict__ values, const DescHead* __restrict__ desc,
     ColState* __restrict__ cs, uint32_t* __restrict__ status0,
-    uint64_t* __restrict__ tail_status, uint32_t* __restrict__ tile_runs) {
+    uint64_t* __restrict__ tail_status, uint32_t* __restrict__ tile_runs, uint32_t sort_tile) {
   __shared__ uint32_t h[4 * 256];
   __shared__ uint32_t hr[4 * 256];
   __shared__ uint32_t s_max[kWaves];
@@ -228,7 +228,7 @@ __global__ __launch_bounds__(kBlock) void k_histogram(
     if (t == 0) {
       uint32_t r = 0;
       for (int i = 0; i < kWaves; ++i) r += s_reps[i];
-      atomicAdd(&tile_runs[cd.sort_tile0 + (uint32_t)(lo / kSortTile)], cnt - r);
+      atomicAdd(&tile_runs[cd.sort_tile0 + (uint32_t)(lo / sort_tile)], cnt - r);
     }
     __syncthreads();
   }
@@ -243,7 +243,7 @@ __global__ __launch_bounds__(kBlock) void k_plan(const DescHead* __restrict__ de
                                                  ColState* __restrict__ cs, Header* __restrict__ hdr,
                                                  uint32_t* __restrict__ tile_runs,
                                                  wahgpu_counts_dev* __restrict__ counts,
-                                                 int force_pair) {
+                                                 int force_pair, uint32_t sort_tile) {
   __shared__ uint64_t s_scan[kWaves];
   __shared__ int s_trivial[4];
   __shared__ uint32_t s_reps;
@@ -266,7 +266,7 @@ __global__ __launch_bounds__(kBlock) void k_plan(const DescHead* __restrict__ de
       // where each radix tile's runs go: exclusive prefix of the runs per tile
       const Column cd = desc_cols(desc)[c];
       uint32_t* tr = tile_runs + cd.sort_tile0;
-      const uint32_t tiles = sort_tiles_of(n);
+      const uint32_t tiles = sort_tiles_of(n, sort_tile);
       uint64_t carry = 0;
       for (uint32_t b = 0; b < tiles; b += kBlock) {
         const uint32_t x = b + t < tiles ? tr[b + t] : 0u;
@@ -300,7 +300,7 @@ __global__ __launch_bounds__(kBlock) void k_plan(const DescHead* __restrict__ de
       wahgpu_counts_dev z;
       z.k = 0; z.length = 0; z.keycnt = 0;
       z.bound = col->maxval; z.passes = a; z.fault = 0; z.mode = run_mode ? kModeRun : kModePair;
-      counts[c] = z;
+      counts[desc_cols(desc)[c].index] = z;
     }
     __syncthreads();
   }
@@ -782,7 +782,7 @@ __global__ __launch_bounds__(THREADS, 4) void k_tail(
     if (t == 0) {
       uint64_t sum = 0;
       for (int i = 0; i < WAVES; ++i) sum += s_ent[i];
-      if (sum) atomicAdd((unsigned long long*)&counts[ent_col].k, (unsigned long long)sum);
+      if (sum) atomicAdd((unsigned long long*)&counts[cols[ent_col].index].k, (unsigned long long)sum);
     }
     entries = 0;
   };
@@ -1014,8 +1014,8 @@ __global__ __launch_bounds__(THREADS, 4) void k_tail(
       else atomicOr(fault, 2u);
     }
     if (base + TILE >= n && t == 0) {  // the column's last tile
-      counts[c].length = (uint64_t)Wpre + Wt;
-      counts[c].keycnt = (uint64_t)Kpre + (tot >> 16);
+      counts[cd.index].length = (uint64_t)Wpre + Wt;
+      counts[cd.index].keycnt = (uint64_t)Kpre + (tot >> 16);
     }
     __syncthreads();
   }
@@ -1036,15 +1036,15 @@ __global__ __launch_bounds__(kBlock) void k_finish(const DescHead* __restrict__ 
   const uint64_t gs = (uint64_t)gridDim.x * kBlock;
   for (uint32_t c = 0; c < ncols; ++c) {
     const Column cd = cols[c];
-    const uint64_t kc = counts[c].keycnt;
-    const uint64_t K = kc < cd.n ? kc : cd.n, L = counts[c].length;
+    const uint64_t kc = counts[cd.index].keycnt;
+    const uint64_t K = kc < cd.n ? kc : cd.n, L = counts[cd.index].length;
     const uint32_t* o = offsets + cd.off;
     uint32_t* len = lengths + cd.off;
     for (uint64_t i = gt; i < K; i += gs) {
       const uint32_t end = (i + 1 < K) ? o[i + 1] : (uint32_t)L;
       len[i] = end - o[i];
     }
-    if (gt == 0 && fault) counts[c].fault = fault;
+    if (gt == 0 && fault) counts[cd.index].fault = fault;
   }
 }
 
@@ -1338,47 +1338,52 @@ static uint32_t cap_grid(uint32_t work, uint32_t cap) {
   return g ? g : 1u;
 }
 
+// The run cut and the four radix launches for one tile shape.
+template <int T, int I>
+static void launch_radix(const BuildArgs& a, hipStream_t s, LaunchHook* hook) {
+  const uint32_t grid =
+      cap_grid(a.sort_tiles_cap, WAH_STATIC_TILES ? a.sort_tiles_cap : a.max_resident * 256 / T);
+  // the cut needs half the registers of a radix pass: two blocks per CU
+  const uint32_t cut_grid = cap_grid(a.sort_tiles_cap, 2 * grid);
+  hipLaunchKernelGGL((k_runcut<T, I>), dim3(cut_grid), dim3(T), 0, s, a.values, a.keysB, a.ridsB,
+                     a.litsB, a.desc, a.cs, a.hdr, a.tile_runs, tile_lanes(cut_grid));
+  if (hook) hook->mark(s);
+  // each launch runs the a-th pass of pair-mode columns, then of run-mode
+  // columns; a kernel with no such column returns at once
+  for (uint32_t p = 0; p < 4; ++p) {
+    if (a.atomic_rank) {
+      hipLaunchKernelGGL((k_onesweep<T, I, false, kRankAtomic>), dim3(grid), dim3(T), 0, s, p,
+                         a.values, a.keysA, a.ridsA, a.keysB, a.ridsB, a.litsA, a.litsB, a.desc,
+                         a.cs, a.hdr, a.sort_status, a.sort_status_stride, tile_lanes(grid));
+      hipLaunchKernelGGL((k_onesweep<T, I, true, kRankAtomic>), dim3(grid), dim3(T), 0, s, p,
+                         a.values, a.keysA, a.ridsA, a.keysB, a.ridsB, a.litsA, a.litsB, a.desc,
+                         a.cs, a.hdr, a.sort_status, a.sort_status_stride, tile_lanes(grid));
+    } else {
+      hipLaunchKernelGGL((k_onesweep<T, I, false, kRankMatch>), dim3(grid), dim3(T), 0, s, p,
+                         a.values, a.keysA, a.ridsA, a.keysB, a.ridsB, a.litsA, a.litsB, a.desc,
+                         a.cs, a.hdr, a.sort_status, a.sort_status_stride, tile_lanes(grid));
+      hipLaunchKernelGGL((k_onesweep<T, I, true, kRankMatch>), dim3(grid), dim3(T), 0, s, p,
+                         a.values, a.keysA, a.ridsA, a.keysB, a.ridsB, a.litsA, a.litsB, a.desc,
+                         a.cs, a.hdr, a.sort_status, a.sort_status_stride, tile_lanes(grid));
+    }
+    if (hook) hook->mark(s);
+  }
+}
+
 void launch_sort(const BuildArgs& a, hipStream_t s, LaunchHook* hook) {
   hipLaunchKernelGGL(k_begin, dim3(64), dim3(kBlock), 0, s, a.hdr, a.cs, a.cols_cap,
                      a.tile_runs, a.sort_tiles_cap, a.desc, (int)a.inline_desc, a.one_dims, a.one);
   uint64_t hb = (a.n_cap + 4 * kHistChunk - 1) / (4 * kHistChunk);
   if (hb > a.hist_chunks_cap) hb = a.hist_chunks_cap;
   hipLaunchKernelGGL(k_histogram, dim3(cap_grid((uint32_t)hb, kHistMaxBlocks)), dim3(kBlock), 0,
-                     s, a.values, a.desc, a.cs, a.sort_status, a.tail_status, a.tile_runs);
+                     s, a.values, a.desc, a.cs, a.sort_status, a.tail_status, a.tile_runs,
+                     radix_tile(a.shape));
   hipLaunchKernelGGL(k_plan, dim3(cap_grid(a.cols_cap, 1024)), dim3(kBlock), 0, s, a.desc, a.cs,
-                     a.hdr, a.tile_runs, a.counts, (int)a.force_pair);
-  const uint32_t grid =
-      cap_grid(a.sort_tiles_cap, WAH_STATIC_TILES ? a.sort_tiles_cap
-                                                  : a.max_resident * 256 / kSortThreads);
-  // the cut needs half the registers of a radix pass: two blocks per CU
-  const uint32_t cut_grid = cap_grid(a.sort_tiles_cap, 2 * grid);
-  hipLaunchKernelGGL((k_runcut<kSortThreads, kSortItems>), dim3(cut_grid), dim3(kSortThreads), 0,
-                     s, a.values, a.keysB, a.ridsB, a.litsB, a.desc, a.cs, a.hdr, a.tile_runs,
-                     tile_lanes(cut_grid));
-  if (hook) hook->mark(s);
-  // each launch runs the a-th pass of pair-mode columns, then of run-mode
-  // columns; a kernel with no such column returns at once
-  for (uint32_t p = 0; p < 4; ++p) {
-    if (a.atomic_rank) {
-      hipLaunchKernelGGL((k_onesweep<kSortThreads, kSortItems, false, kRankAtomic>), dim3(grid),
-                         dim3(kSortThreads), 0, s, p, a.values, a.keysA, a.ridsA, a.keysB,
-                         a.ridsB, a.litsA, a.litsB, a.desc, a.cs, a.hdr, a.sort_status,
-                         a.sort_status_stride, tile_lanes(grid));
-      hipLaunchKernelGGL((k_onesweep<kSortThreads, kSortItems, true, kRankAtomic>), dim3(grid),
-                         dim3(kSortThreads), 0, s, p, a.values, a.keysA, a.ridsA, a.keysB,
-                         a.ridsB, a.litsA, a.litsB, a.desc, a.cs, a.hdr, a.sort_status,
-                         a.sort_status_stride, tile_lanes(grid));
-    } else {
-      hipLaunchKernelGGL((k_onesweep<kSortThreads, kSortItems, false, kRankMatch>), dim3(grid),
-                         dim3(kSortThreads), 0, s, p, a.values, a.keysA, a.ridsA, a.keysB,
-                         a.ridsB, a.litsA, a.litsB, a.desc, a.cs, a.hdr, a.sort_status,
-                         a.sort_status_stride, tile_lanes(grid));
-      hipLaunchKernelGGL((k_onesweep<kSortThreads, kSortItems, true, kRankMatch>), dim3(grid),
-                         dim3(kSortThreads), 0, s, p, a.values, a.keysA, a.ridsA, a.keysB,
-                         a.ridsB, a.litsA, a.litsB, a.desc, a.cs, a.hdr, a.sort_status,
-                         a.sort_status_stride, tile_lanes(grid));
-    }
-    if (hook) hook->mark(s);
+                     a.hdr, a.tile_runs, a.counts, (int)a.force_pair, radix_tile(a.shape));
+  switch (a.shape) {
+    case 1: launch_radix<512, kSortItems>(a, s, hook); break;
+    case 2: launch_radix<256, kSortItems>(a, s, hook); break;
+    default: launch_radix<kSortThreads, kSortItems>(a, s, hook); break;
   }
 }
 

@@ -7,7 +7,9 @@
 #include <stdlib.h>
 #include <string.h>
 
+#include <algorithm>
 #include <chrono>
+#include <cmath>
 #include <string>
 #include <vector>
 
@@ -43,6 +45,10 @@ struct wahgpu_ctx {
   std::vector<uintptr_t> gkey;  // buffers and sizes the graph was captured with
   uint64_t captures = 0, replays = 0;
   bool atomic_rank = false;  // radix ranking by LDS fetch-add (self-tested)
+  // radix tile shape per column-length bucket (wahgpu_autotune / set_tuning)
+  uint32_t tune_shape[WAHGPU_TUNE_BUCKETS] = {0, 0, 0, 0};
+  double tune_ms[WAHGPU_TUNE_BUCKETS][WAHGPU_TUNE_SHAPES] = {};
+  int force_shape = -1;  // autotune runs: every column in this shape
 };
 
 namespace {
@@ -113,16 +119,29 @@ struct Caps {
            tail_tiles >= o.tail_tiles && hist_chunks >= o.hist_chunks;
   }
 };
-Caps caps_one(uint64_t n) {
-  return Caps{n, 1u, wah::sort_tiles_of(n), wah::tail_tiles_of(n), wah::hist_chunks_of(n)};
+// Column-length bucket of n: < 2^14, < 2^17, < 2^20, the rest.
+constexpr uint32_t kBucketLog2[WAHGPU_TUNE_BUCKETS] = {14, 17, 20, 64};
+uint32_t bucket_of(uint64_t n) {
+  for (uint32_t b = 0; b + 1 < WAHGPU_TUNE_BUCKETS; ++b)
+    if (n < (1ull << kBucketLog2[b])) return b;
+  return WAHGPU_TUNE_BUCKETS - 1;
+}
+int shape_for(const wahgpu_ctx* c, uint64_t n) {
+  return c->force_shape >= 0 ? c->force_shape : (int)c->tune_shape[bucket_of(n)];
 }
 
-// Bytes of scratch a fused build needs (besides outputs).
-uint64_t build_scratch_bytes(const Caps& k) {
+Caps caps_one(uint64_t n, int shape) {
+  return Caps{n, 1u, wah::sort_tiles_of(n, wah::radix_tile(shape)), wah::tail_tiles_of(n),
+              wah::hist_chunks_of(n)};
+}
+
+// Bytes of scratch a fused build needs (besides outputs), with `ndesc`
+// descriptor regions of k.cols columns each.
+uint64_t build_scratch_bytes(const Caps& k, uint32_t ndesc = 1) {
   const uint64_t nn = k.n ? k.n : 1;
   uint64_t b = 0;
   b += 6 * align_up(nn * 4);                       // keys, rids and lits of A and B
-  b += align_up(sizeof(wah::DescHead) + (uint64_t)k.cols * sizeof(wah::Column));
+  b += ndesc * align_up(sizeof(wah::DescHead) + (uint64_t)k.cols * sizeof(wah::Column));
   b += align_up(sizeof(wah::Header));
   b += align_up((uint64_t)k.cols * sizeof(wah::ColState));
   b += align_up(2ull * k.sort_tiles * 256 * 4);
@@ -131,7 +150,9 @@ uint64_t build_scratch_bytes(const Caps& k) {
   return b;
 }
 
-void carve_build(wahgpu_ctx* c, BuildArgs* a, const Caps& k) {
+// Carves the shared scratch and the first descriptor region; further regions
+// (one per sub-batch) follow with carve_desc.
+void carve_build(wahgpu_ctx* c, BuildArgs* a, const Caps& k, int shape) {
   const uint64_t nn = k.n ? k.n : 1;
   a->keysA = take<uint32_t>(c, nn);
   a->ridsA = take<uint32_t>(c, nn);
@@ -149,18 +170,24 @@ void carve_build(wahgpu_ctx* c, BuildArgs* a, const Caps& k) {
   a->tile_runs = take<uint32_t>(c, (uint64_t)k.sort_tiles + 1);
   a->max_resident = c->max_resident;
   a->atomic_rank = c->atomic_rank;
+  a->shape = shape;
   a->n_cap = k.n;
   a->cols_cap = k.cols;
   a->sort_tiles_cap = k.sort_tiles;
   a->tail_tiles_cap = k.tail_tiles;
   a->hist_chunks_cap = k.hist_chunks;
 }
+wah::DescHead* carve_desc(wahgpu_ctx* c, uint32_t cols) {
+  return reinterpret_cast<wah::DescHead*>(
+      take<char>(c, sizeof(wah::DescHead) + (uint64_t)cols * sizeof(wah::Column)));
+}
 
 // A single column at offset 0: its descriptor travels in k_begin's arguments.
 void set_one(BuildArgs* a, uint64_t n, uint32_t rid_base) {
   a->inline_desc = true;
-  a->one_dims = wah::Dims{1u, wah::sort_tiles_of(n), wah::tail_tiles_of(n), wah::hist_chunks_of(n)};
-  a->one = wah::Column{0, n, rid_base, 0u, 0u, 0u};
+  a->one_dims = wah::Dims{1u, wah::sort_tiles_of(n, wah::radix_tile(a->shape)),
+                          wah::tail_tiles_of(n), wah::hist_chunks_of(n)};
+  a->one = wah::Column{0, n, rid_base, 0u, 0u, 0u, 0u, {0u, 0u, 0u}};
 }
 
 // Scan scratch, zeroed before use: tile-counter lanes (kLaneWords words),
@@ -399,7 +426,7 @@ static int build_device_impl(wahgpu_ctx* c, const uint32_t* d_values, uint64_t n
                              uint32_t* d_lengths, uint32_t* d_words, wahgpu_counts* d_counts,
                              Marks* marks) {
   BuildArgs a{};
-  carve_build(c, &a, caps_one(n));
+  carve_build(c, &a, caps_one(n, shape_for(c, n)), shape_for(c, n));
   set_one(&a, n, rid_base);
   a.values = d_values;
   a.out_keys = d_keys;
@@ -465,7 +492,7 @@ int wahgpu_build_device(wahgpu_ctx* c, const uint32_t* d_values, uint64_t n, uin
   int rc = check_n(c, n, rid_base);
   if (rc) return rc;
   HIPCHK(c, hipSetDevice(c->device));
-  rc = arena_reserve(c, build_scratch_bytes(caps_one(n)));
+  rc = arena_reserve(c, build_scratch_bytes(caps_one(n, shape_for(c, n))));
   if (rc) return rc;
   return build_device_impl(c, d_values, n, rid_base, d_keys, d_offsets, d_lengths, d_words,
                            d_counts, nullptr);
@@ -481,7 +508,7 @@ int wahgpu_build(wahgpu_ctx* c, const uint32_t* values, uint64_t n, uint32_t rid
   if (rc) return rc;
   HIPCHK(c, hipSetDevice(c->device));
   uint64_t nn = n ? n : 1;
-  uint64_t bytes = build_scratch_bytes(caps_one(n)) + align_up(nn * 4) * 4 + align_up(2 * nn * 4) +
+  uint64_t bytes = build_scratch_bytes(caps_one(n, shape_for(c, n))) + align_up(nn * 4) * 4 + align_up(2 * nn * 4) +
                    align_up(sizeof(wahgpu_counts));
   rc = arena_reserve(c, bytes);
   if (rc) return rc;
@@ -536,14 +563,23 @@ int wahgpu_build(wahgpu_ctx* c, const uint32_t* values, uint64_t n, uint32_t rid
 
 namespace {
 
-// Validates the columns and fills the descriptor (dims + columns with tile
-// and chunk prefixes) into host memory.
-int make_desc(wahgpu_ctx* c, const wahgpu_column* cols, uint32_t ncols, uint64_t values_len,
-              std::vector<char>* out, Caps* exact) {
-  out->assign(sizeof(wah::DescHead) + (uint64_t)ncols * sizeof(wah::Column), 0);
-  auto* head = reinterpret_cast<wah::DescHead*>(out->data());
-  wah::Column* dc = wah::desc_cols(head);
-  uint64_t st = 0, tt = 0, hc = 0, end = 0;
+// One launch set of a batch: the columns whose length bucket maps to one
+// radix tile shape, with their descriptor (dims + columns with tile and
+// chunk prefixes in that shape's tiles) in host memory.
+struct SubBatch {
+  int shape;
+  std::vector<char> desc;
+  Caps caps;
+};
+
+// Validates the columns and splits them into sub-batches by tuned shape
+// (every compiled shape when `all_shapes`, empty ones included, so that a
+// captured graph fits any batch).  *shared = element-wise maximum of the
+// sub-batches' caps (the scratch they share, one after another).
+int plan_batch(wahgpu_ctx* c, const wahgpu_column* cols, uint32_t ncols, uint64_t values_len,
+               bool all_shapes, std::vector<SubBatch>* subs, Caps* shared) {
+  uint64_t end = 0;
+  std::vector<uint32_t> members[wah::kRadixShapes];
   for (uint32_t i = 0; i < ncols; ++i) {
     const wahgpu_column& col = cols[i];
     int rc = check_n(c, col.n, col.rid_base);
@@ -552,15 +588,40 @@ int make_desc(wahgpu_ctx* c, const wahgpu_column* cols, uint32_t ncols, uint64_t
       return fail(c, WAHGPU_EINVAL, "column " + std::to_string(i) +
                                         " overlaps the previous one or exceeds values_len");
     end = col.offset + col.n;
-    dc[i] = wah::Column{col.offset, col.n, col.rid_base, (uint32_t)st, (uint32_t)tt, (uint32_t)hc};
-    st += wah::sort_tiles_of(col.n);
-    tt += wah::tail_tiles_of(col.n);
-    hc += wah::hist_chunks_of(col.n);
+    members[shape_for(c, col.n)].push_back(i);
   }
-  if (st > 0xFFFFFFF0ull || hc > 0xFFFFFFF0ull)
-    return fail(c, WAHGPU_ERANGE, "batch too large for 32-bit tile numbering");
-  head->dims = wah::Dims{ncols, (uint32_t)st, (uint32_t)tt, (uint32_t)hc};
-  *exact = Caps{values_len, ncols, (uint32_t)st, (uint32_t)tt, (uint32_t)hc};
+  subs->clear();
+  *shared = Caps{values_len, 0, 0, 0, 0};
+  bool tuned[wah::kRadixShapes] = {};
+  for (uint32_t b = 0; b < WAHGPU_TUNE_BUCKETS; ++b) tuned[c->tune_shape[b]] = true;
+  for (int sh = 0; sh < wah::kRadixShapes; ++sh) {
+    if (members[sh].empty() && !(all_shapes && tuned[sh])) continue;
+    SubBatch sb;
+    sb.shape = sh;
+    const uint32_t m = (uint32_t)members[sh].size();
+    sb.desc.assign(sizeof(wah::DescHead) + (uint64_t)m * sizeof(wah::Column), 0);
+    auto* head = reinterpret_cast<wah::DescHead*>(sb.desc.data());
+    wah::Column* dc = wah::desc_cols(head);
+    uint64_t st = 0, tt = 0, hc = 0;
+    const uint32_t tile = wah::radix_tile(sh);
+    for (uint32_t j = 0; j < m; ++j) {
+      const wahgpu_column& col = cols[members[sh][j]];
+      dc[j] = wah::Column{col.offset, col.n, col.rid_base, (uint32_t)st, (uint32_t)tt,
+                          (uint32_t)hc, members[sh][j], {0u, 0u, 0u}};
+      st += wah::sort_tiles_of(col.n, tile);
+      tt += wah::tail_tiles_of(col.n);
+      hc += wah::hist_chunks_of(col.n);
+    }
+    if (st > 0xFFFFFFF0ull || hc > 0xFFFFFFF0ull)
+      return fail(c, WAHGPU_ERANGE, "batch too large for 32-bit tile numbering");
+    head->dims = wah::Dims{m, (uint32_t)st, (uint32_t)tt, (uint32_t)hc};
+    sb.caps = Caps{values_len, m, (uint32_t)st, (uint32_t)tt, (uint32_t)hc};
+    shared->cols = std::max(shared->cols, m);
+    shared->sort_tiles = std::max(shared->sort_tiles, (uint32_t)st);
+    shared->tail_tiles = std::max(shared->tail_tiles, (uint32_t)tt);
+    shared->hist_chunks = std::max(shared->hist_chunks, (uint32_t)hc);
+    subs->push_back(std::move(sb));
+  }
   return WAHGPU_OK;
 }
 
@@ -586,41 +647,57 @@ int upload_desc(wahgpu_ctx* c, const std::vector<char>& d, wah::DescHead* dst) {
 // Caps of the reserved layout for graph replay.
 Caps reserved_caps(const wahgpu_ctx* c) {
   const uint64_t v = c->res_values;
-  return Caps{v, c->res_cols, (uint32_t)(v / wah::kSortTile + c->res_cols),
+  uint32_t min_tile = wah::radix_tile(0);
+  for (int sh = 1; sh < wah::kRadixShapes; ++sh) min_tile = std::min(min_tile, wah::radix_tile(sh));
+  return Caps{v, c->res_cols, (uint32_t)(v / min_tile + c->res_cols),
               (uint32_t)(v / wah::kTailTile + c->res_cols),
               (uint32_t)(v / wah::kHistChunk + c->res_cols)};
+}
+
+// Scratch bytes of a planned batch (shared scratch + one descriptor region
+// per sub-batch).
+uint64_t batch_scratch_bytes(const Caps& shared, size_t nsubs) {
+  return build_scratch_bytes(shared, (uint32_t)(nsubs ? nsubs : 1));
 }
 
 int batch_device_impl(wahgpu_ctx* c, const uint32_t* d_values, uint64_t values_len,
                       const wahgpu_column* cols, uint32_t ncols, uint32_t* d_keys,
                       uint32_t* d_offsets, uint32_t* d_lengths, uint32_t* d_words,
                       wahgpu_counts* d_counts, uint64_t extra_bytes) {
-  std::vector<char> desc;
-  Caps exact;
-  int rc = make_desc(c, cols, ncols, values_len, &desc, &exact);
+  std::vector<SubBatch> subs;
+  Caps shared;
+  const bool reserve = c->res_cols != 0 && c->force_shape < 0;
+  int rc = plan_batch(c, cols, ncols, values_len, reserve, &subs, &shared);
   if (rc) return rc;
-  const bool graph = c->res_cols && reserved_caps(c).covers(exact);
-  const Caps k = graph ? reserved_caps(c) : exact;
+  const bool graph = reserve && reserved_caps(c).covers(shared) && ncols <= c->res_cols;
+  const Caps k = graph ? reserved_caps(c) : shared;
   // extra_bytes: room the caller carves after the scratch (host batches)
-  rc = arena_reserve(c, build_scratch_bytes(k) + extra_bytes);
+  rc = arena_reserve(c, batch_scratch_bytes(k, subs.size()) + extra_bytes);
   if (rc) return rc;
-  BuildArgs a{};
-  carve_build(c, &a, k);
-  a.inline_desc = false;
-  a.values = d_values;
-  a.out_keys = d_keys;
-  a.out_offsets = d_offsets;
-  a.out_lengths = d_lengths;
-  a.out_words = d_words;
-  a.counts = d_counts;
-  rc = upload_desc(c, desc, a.desc);
-  if (rc) return rc;
+  BuildArgs base{};
+  carve_build(c, &base, k, 0);
+  std::vector<BuildArgs> args;
+  for (size_t i = 0; i < subs.size(); ++i) {
+    BuildArgs a = base;
+    a.shape = subs[i].shape;
+    if (i > 0) a.desc = carve_desc(c, k.cols);
+    a.inline_desc = false;
+    a.values = d_values;
+    a.out_keys = d_keys;
+    a.out_offsets = d_offsets;
+    a.out_lengths = d_lengths;
+    a.out_words = d_words;
+    a.counts = d_counts;
+    rc = upload_desc(c, subs[i].desc, a.desc);
+    if (rc) return rc;
+    args.push_back(a);
+  }
   if (graph) {
     std::vector<uintptr_t> key = {(uintptr_t)c->arena, (uintptr_t)c->stream,
                                   (uintptr_t)d_values, (uintptr_t)d_keys,
                                   (uintptr_t)d_offsets, (uintptr_t)d_lengths,
                                   (uintptr_t)d_words, (uintptr_t)d_counts,
-                                  (uintptr_t)k.n, (uintptr_t)k.cols};
+                                  (uintptr_t)k.n, (uintptr_t)k.cols, (uintptr_t)subs.size()};
     if (!c->gexec || key != c->gkey) {
       if (c->gexec) {
         HIPCHK(c, hipStreamSynchronize(c->stream));
@@ -628,8 +705,10 @@ int batch_device_impl(wahgpu_ctx* c, const uint32_t* d_values, uint64_t values_l
         c->gexec = nullptr;
       }
       HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
-      wah::launch_sort(a, c->stream);
-      wah::launch_tail(a, c->stream);
+      for (const BuildArgs& a : args) {
+        wah::launch_sort(a, c->stream);
+        wah::launch_tail(a, c->stream);
+      }
       hipGraph_t g = nullptr;
       hipError_t e = hipStreamEndCapture(c->stream, &g);
       if (e != hipSuccess) return fail(c, WAHGPU_EHIP, std::string("capture: ") + hipGetErrorString(e));
@@ -646,19 +725,21 @@ int batch_device_impl(wahgpu_ctx* c, const uint32_t* d_values, uint64_t values_l
     ++c->replays;
     return WAHGPU_OK;
   }
-  EventHook hook;
-  EventHook* hp = nullptr;
-  if (c->timing && c->tcalls < c->tmax) {
-    hook.ev = &c->tev[(size_t)c->tcalls * (WAHGPU_TIMING_GROUPS + 1)];
-    hp = &hook;
-    ++c->tcalls;
-    hook.mark(c->stream);
+  for (const BuildArgs& a : args) {
+    EventHook hook;
+    EventHook* hp = nullptr;
+    if (c->timing && c->tcalls < c->tmax && args.size() == 1) {
+      hook.ev = &c->tev[(size_t)c->tcalls * (WAHGPU_TIMING_GROUPS + 1)];
+      hp = &hook;
+      ++c->tcalls;
+      hook.mark(c->stream);
+    }
+    wah::launch_sort(a, c->stream, hp);
+    LAUNCHCHK(c);
+    wah::launch_tail(a, c->stream, hp);
+    LAUNCHCHK(c);
+    if (hp) hook.mark(c->stream);
   }
-  wah::launch_sort(a, c->stream, hp);
-  LAUNCHCHK(c);
-  wah::launch_tail(a, c->stream, hp);
-  LAUNCHCHK(c);
-  if (hp) hook.mark(c->stream);
   return WAHGPU_OK;
 }
 
@@ -697,12 +778,13 @@ int wahgpu_build_batch(wahgpu_ctx* c, const uint32_t* const* values, const uint6
                       align_up((uint64_t)ncols * sizeof(wahgpu_counts));
   // The scratch is carved first by batch_device_impl; the io buffers follow
   // it.  Reserve both, then carve io at the end of the scratch.
-  std::vector<char> desc;
-  Caps exact;
-  int rc = make_desc(c, cols.data(), ncols, total, &desc, &exact);
+  std::vector<SubBatch> subs;
+  Caps shared;
+  const bool reserve = c->res_cols != 0 && c->force_shape < 0;
+  int rc = plan_batch(c, cols.data(), ncols, total, reserve, &subs, &shared);
   if (rc) return rc;
-  const bool graph = c->res_cols && reserved_caps(c).covers(exact);
-  const uint64_t scratch = build_scratch_bytes(graph ? reserved_caps(c) : exact);
+  const bool graph = reserve && reserved_caps(c).covers(shared) && ncols <= c->res_cols;
+  const uint64_t scratch = batch_scratch_bytes(graph ? reserved_caps(c) : shared, subs.size());
   rc = arena_reserve(c, scratch + io);
   if (rc) return rc;
   c->bump = scratch;
@@ -770,6 +852,111 @@ int wahgpu_batch_reserve(wahgpu_ctx* c, uint64_t max_values, uint32_t max_cols) 
   return WAHGPU_OK;
 }
 
+int wahgpu_get_tuning(const wahgpu_ctx* c, wahgpu_tuning* out) {
+  if (!c || !out) return WAHGPU_EINVAL;
+  for (uint32_t b = 0; b < WAHGPU_TUNE_BUCKETS; ++b) {
+    out->bucket_log2[b] = kBucketLog2[b];
+    out->shape[b] = c->tune_shape[b];
+    out->digit_bits[b] = 8;
+    for (uint32_t s = 0; s < WAHGPU_TUNE_SHAPES; ++s) out->ms[b][s] = c->tune_ms[b][s];
+  }
+  return WAHGPU_OK;
+}
+
+static void drop_graph(wahgpu_ctx* c) {
+  if (c->gexec) {
+    (void)hipStreamSynchronize(c->stream);
+    (void)hipGraphExecDestroy(c->gexec);
+    c->gexec = nullptr;
+    c->gkey.clear();
+  }
+}
+
+int wahgpu_set_tuning(wahgpu_ctx* c, const wahgpu_tuning* in) {
+  if (!c || !in) return WAHGPU_EINVAL;
+  for (uint32_t b = 0; b < WAHGPU_TUNE_BUCKETS; ++b)
+    if (in->shape[b] >= WAHGPU_TUNE_SHAPES) return fail(c, WAHGPU_EINVAL, "unknown tile shape");
+  HIPCHK(c, hipSetDevice(c->device));
+  drop_graph(c);  // the set of launch sets a batch runs may change
+  for (uint32_t b = 0; b < WAHGPU_TUNE_BUCKETS; ++b) c->tune_shape[b] = in->shape[b];
+  return WAHGPU_OK;
+}
+
+// Times every tile shape on a synthetic batch of each length bucket and keeps
+// the fastest: columns of one length per bucket (2^13, 2^15.5, 2^18.5, 2^22
+// values), uniform over 4096 values, about 2^24 values per bucket.
+int wahgpu_autotune(wahgpu_ctx* c, wahgpu_tuning* out) {
+  if (!c) return WAHGPU_EINVAL;
+  HIPCHK(c, hipSetDevice(c->device));
+  drop_graph(c);
+  const double len_log2[WAHGPU_TUNE_BUCKETS] = {13.0, 15.5, 18.5, 22.0};
+  const uint64_t total = 1ull << 24;
+  uint32_t *d_v = nullptr, *d_k = nullptr, *d_o = nullptr, *d_l = nullptr, *d_w = nullptr;
+  wahgpu_counts* d_c = nullptr;
+  const uint64_t maxcols = 4096;
+  int rc = WAHGPU_OK;
+  hipEvent_t ev[2] = {nullptr, nullptr};
+  std::vector<uint32_t> host(total);
+  auto cleanup = [&]() {
+    if (d_v) (void)hipFree(d_v);
+    if (d_k) (void)hipFree(d_k);
+    if (d_o) (void)hipFree(d_o);
+    if (d_l) (void)hipFree(d_l);
+    if (d_w) (void)hipFree(d_w);
+    if (d_c) (void)hipFree(d_c);
+    for (auto& e : ev)
+      if (e) (void)hipEventDestroy(e);
+    c->force_shape = -1;
+  };
+  if (hipMalloc(&d_v, total * 4) != hipSuccess || hipMalloc(&d_k, total * 4) != hipSuccess ||
+      hipMalloc(&d_o, total * 4) != hipSuccess || hipMalloc(&d_l, total * 4) != hipSuccess ||
+      hipMalloc(&d_w, 2 * total * 4) != hipSuccess ||
+      hipMalloc(&d_c, maxcols * sizeof(wahgpu_counts)) != hipSuccess ||
+      hipEventCreate(&ev[0]) != hipSuccess || hipEventCreate(&ev[1]) != hipSuccess) {
+    (void)hipGetLastError();
+    cleanup();
+    return fail(c, WAHGPU_ENOMEM, "autotune: device allocation failed");
+  }
+  for (uint32_t b = 0; b < WAHGPU_TUNE_BUCKETS && rc == WAHGPU_OK; ++b) {
+    const uint64_t len = (uint64_t)std::pow(2.0, len_log2[b]);
+    const uint64_t step = (len + 63) / 64 * 64;
+    const uint32_t ncols = (uint32_t)std::min<uint64_t>(maxcols, total / step);
+    std::vector<wahgpu_column> cols(ncols);
+    for (uint32_t i = 0; i < ncols; ++i) cols[i] = wahgpu_column{i * step, len, 0u, 0u};
+    wahgpu_gen_params gp{};
+    gp.dist = WAHGPU_DIST_UNIFORM;
+    gp.from = 0;
+    gp.to = 4095;
+    gp.seed = 1000 + b;
+    if (wahgpu_generate(&gp, host.data(), total) != WAHGPU_OK) rc = WAHGPU_EINVAL;
+    if (rc == WAHGPU_OK &&
+        hipMemcpy(d_v, host.data(), total * 4, hipMemcpyHostToDevice) != hipSuccess)
+      rc = WAHGPU_EHIP;
+    double best = 1e30;
+    for (int sh = 0; sh < WAHGPU_TUNE_SHAPES && rc == WAHGPU_OK; ++sh) {
+      c->force_shape = sh;
+      float ms_best = 1e30f;
+      for (int rep = 0; rep < 4 && rc == WAHGPU_OK; ++rep) {
+        (void)hipEventRecord(ev[0], c->stream);
+        rc = batch_device_impl(c, d_v, total, cols.data(), ncols, d_k, d_o, d_l, d_w, d_c, 0);
+        (void)hipEventRecord(ev[1], c->stream);
+        if (rc == WAHGPU_OK && hipEventSynchronize(ev[1]) != hipSuccess) rc = WAHGPU_EHIP;
+        float ms = 0.f;
+        if (rc == WAHGPU_OK) (void)hipEventElapsedTime(&ms, ev[0], ev[1]);
+        if (rep > 0 && ms < ms_best) ms_best = ms;  // first run warms up
+      }
+      c->tune_ms[b][sh] = ms_best;
+      if (rc == WAHGPU_OK && ms_best < best) {
+        best = ms_best;
+        c->tune_shape[b] = (uint32_t)sh;
+      }
+    }
+  }
+  cleanup();
+  if (rc != WAHGPU_OK) return rc == WAHGPU_EHIP ? fail(c, rc, "autotune: build failed") : rc;
+  return out ? wahgpu_get_tuning(c, out) : WAHGPU_OK;
+}
+
 int wahgpu_graph_stats(const wahgpu_ctx* c, uint64_t* captures, uint64_t* replays) {
   if (!c) return WAHGPU_EINVAL;
   if (captures) *captures = c->captures;
@@ -789,13 +976,13 @@ int wahgpu_build_staged(wahgpu_ctx* c, const uint32_t* values, uint64_t n, uint3
   if (rc) return rc;
   HIPCHK(c, hipSetDevice(c->device));
   uint64_t nn = n ? n : 1;
-  uint64_t bytes = build_scratch_bytes(caps_one(n)) + align_up(nn * 4) * 12 + align_up(2 * nn * 4) +
+  uint64_t bytes = build_scratch_bytes(caps_one(n, shape_for(c, n))) + align_up(nn * 4) * 12 + align_up(2 * nn * 4) +
                    align_up(sizeof(wahgpu_counts)) + 4 * scan_scratch_bytes(2 * nn);
   rc = arena_reserve(c, bytes);
   if (rc) return rc;
   auto t0 = std::chrono::steady_clock::now();
   BuildArgs a{};
-  carve_build(c, &a, caps_one(n));
+  carve_build(c, &a, caps_one(n, shape_for(c, n)), shape_for(c, n));
   set_one(&a, n, 0);
   uint32_t* d_values = take<uint32_t>(c, nn);
   uint32_t* s_keys = take<uint32_t>(c, nn);
@@ -900,10 +1087,10 @@ int wahgpu_sort_rids_by_value(wahgpu_ctx* c, const uint32_t* d_values, uint64_t 
   int rc = check_n(c, n, rid_base);
   if (rc) return rc;
   HIPCHK(c, hipSetDevice(c->device));
-  rc = arena_reserve(c, build_scratch_bytes(caps_one(n)) + align_up(sizeof(wahgpu_counts)));
+  rc = arena_reserve(c, build_scratch_bytes(caps_one(n, shape_for(c, n))) + align_up(sizeof(wahgpu_counts)));
   if (rc) return rc;
   BuildArgs a{};
-  carve_build(c, &a, caps_one(n));
+  carve_build(c, &a, caps_one(n, shape_for(c, n)), shape_for(c, n));
   set_one(&a, n, rid_base);
   a.values = d_values;
   a.counts = take<wahgpu_counts>(c, 1);

@@ -146,3 +146,68 @@ def test_graph_replays_match_eager(ctx):
         assert captures == 1 and replays == 4
     finally:
         g.close()
+
+
+@pytest.mark.parametrize("shape", [0, 1, 2])
+def test_every_tile_shape_matches_oracle(shape):
+    """Each compiled radix tile shape, for single columns and a batch."""
+    c = ia.Context(0)
+    try:
+        c.set_tuning([shape] * 4)
+        assert c.tuning()["shape"] == [shape] * 4
+        rng = np.random.default_rng(40 + shape)
+        singles = [rng.integers(0, 65536, 100003).astype(np.uint32),
+                   rng.integers(0, 1 << 32, 9000, dtype=np.uint64).astype(np.uint32),
+                   ia.generate(200000, "zipf-runs", seed=shape, cardinality=1 << 20,
+                               zipf_s=1.2, run_mean=100)]
+        for i, v in enumerate(singles):
+            print(f"[shape {shape}] single {i} n={v.size}", flush=True)
+            assert_same(c.build(v), oracle.build(v), f"shape {shape} single {i}")
+        cols = mixed_columns(60 + shape, 30, max_log=18)
+        buf, layout = pack(cols, gap=2)
+        out = c.build_batch_device(dev(buf), layout)
+        counts = c.read_counts_batch(out[4], len(cols))
+        got = per_column(out, counts, layout)
+        for i, col in enumerate(cols):
+            assert_same(got[i], oracle.build(col), f"shape {shape} column {i}")
+    finally:
+        c.close()
+
+
+def test_mixed_shapes_in_one_batch_and_graph():
+    """Buckets mapped to different shapes: one launch set per shape, also
+    under graph replay."""
+    import torch
+    c = ia.Context(0)
+    try:
+        c.set_tuning([2, 1, 0, 1])
+        cols = mixed_columns(71, 40, max_log=22)
+        buf, layout = pack(cols)
+        ref = [oracle.build(col) for col in cols]
+        out = c.build_batch_device(dev(buf), layout)
+        got = per_column(out, c.read_counts_batch(out[4], len(cols)), layout)
+        for i in range(len(cols)):
+            assert_same(got[i], ref[i], f"eager column {i}")
+        c.batch_reserve(buf.size, len(cols))
+        d = dev(buf)
+        outs = (c._empty(buf.size), c._empty(buf.size), c._empty(buf.size),
+                c._empty(2 * buf.size), torch.zeros(len(cols) * 10, dtype=torch.int32,
+                                                     device="cuda:0"))
+        for rep in range(3):
+            c.build_batch_device(d, layout, out=outs)
+            got = per_column(outs, c.read_counts_batch(outs[4], len(cols)), layout)
+            for i in range(len(cols)):
+                assert_same(got[i], ref[i], f"replay {rep} column {i}")
+        assert c.graph_stats() == (1, 3)
+    finally:
+        c.close()
+
+
+def test_autotune_reports_a_table(ctx):
+    t = ctx.autotune()
+    print(f"[autotune] {t}", flush=True)
+    assert t["bucket_log2"] == [14, 17, 20, 64] and t["digit_bits"] == [8] * 4
+    for b in range(4):
+        assert t["shape"][b] in (0, 1, 2)
+        assert all(ms > 0 for ms in t["ms"][b])
+        assert t["ms"][b][t["shape"][b]] == min(t["ms"][b])

@@ -207,6 +207,8 @@ def main():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--n", type=int, default=N_VALUES, help="column length (default 2^28)")
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="keep the default tile shapes (profiled runs: only the measured builds)")
     ap.add_argument("--column", choices=["uniform", "zipf-runs"], default="uniform",
                     help="uniform [0,65535] seed 1 (default), or Zipf(1.2) over 2^20 values in "
                          "runs of mean length 4096, seed 1")
@@ -236,7 +238,8 @@ def main():
         column = "Zipf(1.2) over 2^20 values in runs of mean length 4096, seed 1"
     d_values = torch.from_numpy(host.view(np.int32)).to("cuda:0")
     ctx = ia.Context(0)
-    tuning = ctx.autotune()                            # tile shape per length bucket
+    # tile shape per length bucket
+    tuning = ctx.tuning() if args.no_autotune else ctx.autotune()
     out = ctx.build_device(d_values)                   # allocates outputs + arena
     for _ in range(args.warmup):
         ctx.build_device(d_values, out=out)

@@ -61,6 +61,7 @@ struct Slot {
   uint32_t* d_words = nullptr;
   wahgpu_counts* d_counts = nullptr;
   std::vector<wahgpu_counts> counts;
+  std::vector<hipEvent_t> done;  // one per batch column: its result copies ended
   std::thread th;
 };
 
@@ -94,12 +95,31 @@ int run_plain(Slot* sl, Request* r) {
                       nullptr);
 }
 
-// One batch: copy in, batched build (graph replay), counts, copy out.
-// Per-request outcomes go to status[i] (published under the lock once every
-// copy has finished); the return value is a batch-wide failure.
-int run_batch(wahgpu_server* s, Slot* sl, const std::vector<Request*>& batch,
-              std::vector<int>* status) {
+// Completes one request: status and latency under the lock, then wakes the
+// waiters.  Called as each request's result copies end, so the small
+// requests of a batch do not wait for the copies of its large ones.
+void publish(wahgpu_server* s, Request* r, int status) {
+  const Clock::time_point now = Clock::now();
+  {
+    std::lock_guard<std::mutex> lk(s->mu);
+    r->status = status;
+    s->lat_us.push_back(std::chrono::duration<double, std::micro>(now - r->submitted).count());
+    s->values += r->n;
+    --s->pending;
+    s->last_done = now;
+  }
+  s->done_cv.notify_all();
+}
+
+// One batch: copy in, batched build (graph replay), counts, then each
+// request's result copies followed by its event; requests complete in
+// batch order as their events pass.
+void run_batch(wahgpu_server* s, Slot* sl, const std::vector<Request*>& batch) {
   hipStream_t st = (hipStream_t)wahgpu_stream(sl->ctx);
+  auto fail_all = [&](size_t from, int rc) {
+    (void)hipStreamSynchronize(st);  // no copy may still target a caller's buffer
+    for (size_t i = from; i < batch.size(); ++i) publish(s, batch[i], rc);
+  };
   std::vector<wahgpu_column> cols(batch.size());
   uint64_t off = 0;
   for (size_t i = 0; i < batch.size(); ++i) {
@@ -108,56 +128,59 @@ int run_batch(wahgpu_server* s, Slot* sl, const std::vector<Request*>& batch,
     if (r->n &&
         hipMemcpyAsync(sl->d_values + off, r->values, r->n * 4, hipMemcpyHostToDevice, st) !=
             hipSuccess)
-      return WAHGPU_EHIP;
+      return fail_all(0, WAHGPU_EHIP);
     off += align_values(r->n);
   }
   // values_len is the slot capacity, so every batch replays the same graph
   int rc = wahgpu_build_batch_device(sl->ctx, sl->d_values, s->slot_values, cols.data(),
                                      (uint32_t)cols.size(), sl->d_keys, sl->d_offsets,
                                      sl->d_lengths, sl->d_words, sl->d_counts);
-  if (rc != WAHGPU_OK) return rc;
+  if (rc != WAHGPU_OK) return fail_all(0, rc);
   if (hipMemcpyAsync(sl->counts.data(), sl->d_counts, batch.size() * sizeof(wahgpu_counts),
                      hipMemcpyDeviceToHost, st) != hipSuccess ||
       hipStreamSynchronize(st) != hipSuccess)
-    return WAHGPU_EHIP;
-  for (size_t i = 0; i < batch.size(); ++i) {
+    return fail_all(0, WAHGPU_EHIP);
+  std::vector<int> status(batch.size(), WAHGPU_OK);
+  size_t issued = 0;
+  for (; issued < batch.size(); ++issued) {
+    const size_t i = issued;
     Request* r = batch[i];
     const wahgpu_counts& k = sl->counts[i];
     wahgpu_result* res = r->result;
     res->counts = k;
-    if (k.fault) {
-      (*status)[i] = WAHGPU_EHIP;
-      continue;
-    }
-    if (k.keycnt > res->keys_cap || k.length > res->words_cap ||
-        (k.length && !res->words) ||
-        (k.keycnt && (!res->keys || !res->offsets || !res->lengths))) {
-      (*status)[i] = WAHGPU_EINVAL;
-      continue;
-    }
-    const uint64_t o = cols[i].offset;
     bool ok = true;
-    if (k.length)
-      ok &= hipMemcpyAsync(res->words, sl->d_words + 2 * o, k.length * 4, hipMemcpyDeviceToHost,
-                           st) == hipSuccess;
-    if (k.keycnt) {
-      ok &= hipMemcpyAsync(res->keys, sl->d_keys + o, k.keycnt * 4, hipMemcpyDeviceToHost, st) ==
-            hipSuccess;
-      ok &= hipMemcpyAsync(res->offsets, sl->d_offsets + o, k.keycnt * 4, hipMemcpyDeviceToHost,
-                           st) == hipSuccess;
-      ok &= hipMemcpyAsync(res->lengths, sl->d_lengths + o, k.keycnt * 4, hipMemcpyDeviceToHost,
-                           st) == hipSuccess;
+    if (k.fault) {
+      status[i] = WAHGPU_EHIP;
+    } else if (k.keycnt > res->keys_cap || k.length > res->words_cap ||
+               (k.length && !res->words) ||
+               (k.keycnt && (!res->keys || !res->offsets || !res->lengths))) {
+      status[i] = WAHGPU_EINVAL;
+    } else {
+      const uint64_t o = cols[i].offset;
+      if (k.length)
+        ok &= hipMemcpyAsync(res->words, sl->d_words + 2 * o, k.length * 4,
+                             hipMemcpyDeviceToHost, st) == hipSuccess;
+      if (k.keycnt) {
+        ok &= hipMemcpyAsync(res->keys, sl->d_keys + o, k.keycnt * 4, hipMemcpyDeviceToHost,
+                             st) == hipSuccess;
+        ok &= hipMemcpyAsync(res->offsets, sl->d_offsets + o, k.keycnt * 4,
+                             hipMemcpyDeviceToHost, st) == hipSuccess;
+        ok &= hipMemcpyAsync(res->lengths, sl->d_lengths + o, k.keycnt * 4,
+                             hipMemcpyDeviceToHost, st) == hipSuccess;
+      }
     }
-    (*status)[i] = ok ? WAHGPU_OK : WAHGPU_EHIP;
+    if (!ok || hipEventRecord(sl->done[i], st) != hipSuccess) break;
   }
-  if (hipStreamSynchronize(st) != hipSuccess) return WAHGPU_EHIP;
-  return WAHGPU_OK;
+  for (size_t i = 0; i < issued; ++i) {
+    if (hipEventSynchronize(sl->done[i]) != hipSuccess) return fail_all(i, WAHGPU_EHIP);
+    publish(s, batch[i], status[i]);
+  }
+  if (issued < batch.size()) fail_all(issued, WAHGPU_EHIP);
 }
 
 void slot_loop(wahgpu_server* s, Slot* sl) {
   (void)hipSetDevice(s->cfg.device);
   std::vector<Request*> batch;
-  std::vector<int> status;
   for (;;) {
     batch.clear();
     {
@@ -174,26 +197,11 @@ void slot_loop(wahgpu_server* s, Slot* sl) {
         s->queue.pop_front();
         if (s->cfg.plain) break;
       }
-    }
-    status.assign(batch.size(), WAHGPU_OK);
-    int rc = WAHGPU_OK;
-    if (s->cfg.plain) status[0] = run_plain(sl, batch[0]);
-    else rc = run_batch(s, sl, batch, &status);
-    const Clock::time_point now = Clock::now();
-    {
-      std::lock_guard<std::mutex> lk(s->mu);
-      for (size_t i = 0; i < batch.size(); ++i) {
-        Request* r = batch[i];
-        r->status = rc != WAHGPU_OK ? rc : status[i];
-        s->lat_us.push_back(std::chrono::duration<double, std::micro>(now - r->submitted).count());
-        s->values += r->n;
-        --s->pending;
-      }
       ++s->batches;
       s->max_cols = std::max<uint64_t>(s->max_cols, batch.size());
-      s->last_done = now;
     }
-    s->done_cv.notify_all();
+    if (s->cfg.plain) publish(s, batch[0], run_plain(sl, batch[0]));
+    else run_batch(s, sl, batch);
   }
 }
 
@@ -204,6 +212,8 @@ void free_slot(Slot* sl) {
   if (sl->d_lengths) (void)hipFree(sl->d_lengths);
   if (sl->d_words) (void)hipFree(sl->d_words);
   if (sl->d_counts) (void)hipFree(sl->d_counts);
+  for (hipEvent_t e : sl->done) (void)hipEventDestroy(e);
+  sl->done.clear();
   if (sl->ctx) wahgpu_destroy(sl->ctx);
 }
 
@@ -241,6 +251,11 @@ int wahgpu_server_create(wahgpu_server** out, const wahgpu_server_config* cfg) {
            hipMalloc(&sl->d_words, 2 * v * 4) == hipSuccess &&
            hipMalloc(&sl->d_counts, s->cfg.max_batch_cols * sizeof(wahgpu_counts)) == hipSuccess;
       sl->counts.resize(s->cfg.max_batch_cols);
+      for (uint32_t e = 0; ok && e < s->cfg.max_batch_cols; ++e) {
+        hipEvent_t ev = nullptr;
+        ok = hipEventCreateWithFlags(&ev, hipEventDisableTiming) == hipSuccess;
+        if (ok) sl->done.push_back(ev);
+      }
     }
     if (!ok) {
       (void)hipGetLastError();

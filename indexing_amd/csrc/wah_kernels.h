@@ -41,6 +41,17 @@ __host__ __device__ inline uint32_t radix_tile(int shape) {
 constexpr int kTailThreads = WAH_TAIL_THREADS;
 constexpr int kTailItems = WAH_TAIL_ITEMS;
 constexpr int kTailTile = kTailThreads * kTailItems;  // sorted pairs per tail tile
+// Tail occupancy: 8 waves per SIMD (64 VGPRs), so two 1024-thread blocks
+// (2 x 64 KiB of staged words) share a CU and one loads while the other
+// scans, looks back or stores.
+#ifndef WAH_TAIL_LANES  // tile counters of the tail (0: as many as the grid allows, up to 8)
+#define WAH_TAIL_LANES 0
+#endif
+#ifndef WAH_TAIL_WAVES_PER_EU
+#define WAH_TAIL_WAVES_PER_EU 8
+#endif
+constexpr int kTailWavesPerEU = WAH_TAIL_WAVES_PER_EU;
+constexpr int kTailBlocksPerCU = kTailWavesPerEU * 4 * 64 / kTailThreads;
 constexpr int kScanItems = 8;
 constexpr int kScanTile = 256 * kScanItems;
 constexpr uint32_t kHistMaxBlocks = 2048;

@@ -718,32 +718,29 @@ __device__ __forceinline__ SortedView sorted_view(const ColState* col, const Col
   return v;
 }
 
-// Bit sums (positions 0-15 and 16-30 apart) of the lanes before the first
-// entry start of 64 consecutive items; zero when lane 0 starts an entry.
-// b = the lane's literal bits (0: no item), h = the lane starts an entry.
-__device__ __forceinline__ void run_lead(uint32_t b, bool h, uint32_t* lo, uint32_t* hi) {
-  const uint32_t slo = wave_incl_add(b & 0xFFFFu), shi = wave_incl_add(b >> 16);
-  const uint64_t hm = ballot(h);
-  const int fh = hm ? __builtin_ctzll(hm) : kWave;
-  const bool cont = (hm & 1ull) == 0;
-  *lo = cont ? (uint32_t)__builtin_amdgcn_readlane((int)slo, fh - 1) : 0u;
-  *hi = cont ? (uint32_t)__builtin_amdgcn_readlane((int)shi, fh - 1) : 0u;
-}
-
 // ---------------------------------------------------------------------------
 // Fused tail over the sorted items: (value, rid) pairs, or in run mode
-// (value, rid, literal bits) runs.  A tile of THREADS x ITEMS items is staged
-// in LDS; wave w owns items [w*64*ITEMS, +64*ITEMS) in (item, lane) order, so
-// lanes read consecutive LDS words.  Per item: chid = rid/31; an entry starts
-// where (value, chid) changes and ORs the literal bits of the items up to
-// the next entry (at most 31 rows); a key starts where value changes.
-// fill = chid for a key start, else chid - chid_prev - 1; an entry emits
-// [fill if nonzero, literal].  Word and key positions: DPP wave scans of the
-// packed (keys << 16 | words) counts, then a wave-parallel look-back across
-// tiles.  Words are staged in LDS and stored contiguously.
+// (value, rid, literal bits) runs.  Per item: chid = rid/31; an entry starts
+// where (value, chid) changes and ORs the literal bits of the items up to the
+// next entry start (at most 31 rows, one chunk); a key starts where value
+// changes.  fill = chid for a key start, else chid - chid_prev - 1; an entry
+// emits [fill if nonzero, literal].
+//
+// Blocked layout: thread t owns the ITEMS consecutive items [t*ITEMS, +ITEMS)
+// of a tile, loaded from HBM straight into registers with 16-byte loads.  An
+// item's predecessor is the lane's previous register, lane l-1's last item
+// (DPP wave shift) or, for lane 0, one load per wave.  Word and key counts
+// are summed along the lane, then one DPP wave scan and the wave totals give
+// each lane its tile-local offsets; a wave-parallel look-back across tiles
+// gives the tile's.  The run of a lane's last entry start continues into at
+// most four following lanes (31 items, ITEMS >= 8 per lane): their lead bits
+// (before their first entry start) arrive by DPP lane shifts; a run crossing
+// into the next wave or tile takes that wave's or tile's lead bits, which
+// are exchanged in LDS.  Words are staged in LDS at tile-local positions and
+// stored contiguously.
 // ---------------------------------------------------------------------------
 template <int THREADS, int ITEMS, bool RUN>
-__global__ __launch_bounds__(THREADS, 4) void k_tail(
+__global__ __launch_bounds__(THREADS, kTailWavesPerEU) void k_tail(
     const DescHead* __restrict__ desc, const ColState* __restrict__ cs, Header* __restrict__ hdr,
     const uint32_t* values, const uint32_t* keysA, const uint32_t* ridsA,
     const uint32_t* keysB, const uint32_t* ridsB, const uint32_t* litsA, const uint32_t* litsB,
@@ -752,38 +749,26 @@ __global__ __launch_bounds__(THREADS, 4) void k_tail(
     wahgpu_counts_dev* __restrict__ counts) {
   constexpr int TILE = THREADS * ITEMS;
   constexpr int WAVES = THREADS / kWave;
-  constexpr int SUB = kWave * ITEMS;     // pairs per wave
-  constexpr int PAD = 4;                 // s_k[PAD - 1] holds pair base - 1
-  constexpr int HALO = 32;               // pairs read past the tile by runs
-  constexpr int SPAN = TILE + PAD + HALO;
-  static_assert(TILE % 4 == 0 && TILE <= 32768, "word counts fit 16 bits");
-  __shared__ __align__(16) uint32_t s_buf[(RUN ? 3 : 2) * SPAN];
+  static_assert(ITEMS % 4 == 0 && ITEMS >= 8 && ITEMS <= 16, "16-byte loads; runs span <= 4 lanes");
+  static_assert(2 * TILE < 65536, "tile word and key counts fit 16 bits");
+  constexpr uint32_t kAll = (1u << ITEMS) - 1u;
+  __shared__ uint32_t s_words[2 * TILE];
   __shared__ uint32_t s_wtot[WAVES];
-  __shared__ uint32_t s_lead[2 * (WAVES + 1)];
+  __shared__ uint32_t s_wlead[WAVES + 1];  // [w]: lead bits of wave w; [WAVES]: of the next tile
   __shared__ uint64_t s_prefix;
-  __shared__ uint64_t s_ent[WAVES];
   __shared__ uint32_t s_tile;
-  uint32_t* s_k = s_buf;
-  uint32_t* s_r = s_buf + SPAN;
-  uint32_t* s_l = s_buf + 2 * SPAN;  // run mode: literal bits
-  uint32_t* s_words = s_buf;  // reused once the items are consumed (2 * TILE words)
   const Dims dims = desc->dims;
   const Column* cols = desc_cols(desc);
   const int t = threadIdx.x, w = t / kWave, l = t % kWave;
   uint32_t* fault = &hdr->fault;
-  uint64_t entries = 0;           // entries of column ent_col counted by this thread
+  uint32_t entries = 0;           // entries of column ent_col counted by this thread
   uint32_t ent_col = 0xFFFFFFFFu;
   if (hdr->mode_cols[RUN ? 1 : 0] == 0) return;  // no column in this mode
-  // k of a column: one atomic per block and column
+  // k of a column: one atomic per wave and column
   auto flush_entries = [&]() {
-    const uint64_t e = wave_sum64(entries);
-    if (l == 0) s_ent[w] = e;
-    __syncthreads();
-    if (t == 0) {
-      uint64_t sum = 0;
-      for (int i = 0; i < WAVES; ++i) sum += s_ent[i];
-      if (sum) atomicAdd((unsigned long long*)&counts[cols[ent_col].index].k, (unsigned long long)sum);
-    }
+    const uint64_t e = wave_sum64((uint64_t)entries);
+    if (l == 0 && e)
+      atomicAdd((unsigned long long*)&counts[cols[ent_col].index].k, (unsigned long long)e);
     entries = 0;
   };
 
@@ -795,9 +780,10 @@ __global__ __launch_bounds__(THREADS, 4) void k_tail(
     const uint32_t c = find_segment(&cols[0].tail_tile0, sizeof(Column) / 4, dims.ncols, tile);
     const Column cd = cols[c];
     const uint32_t first_tile = cd.tail_tile0;
-    const uint64_t base = (uint64_t)(tile - first_tile) * TILE;  // column-local
+    // column-local indices fit 32 bits: a column holds fewer than 2^30 items
+    const uint32_t base = (tile - first_tile) * (uint32_t)TILE;
     // items: rows, or the runs of a run-mode column (fewer than its tiles)
-    const uint64_t n = RUN ? (uint64_t)cs[c].items : cd.n;
+    const uint32_t n = RUN ? (uint32_t)cs[c].items : (uint32_t)cd.n;
     if (cs[c].mode != (RUN ? kModeRun : kModePair) || base >= n) {
       if (RUN && c + 1 == dims.ncols && cs[c].mode == kModeRun) break;  // past the last items
       __syncthreads();  // s_tile is read by every wave before it is rewritten
@@ -810,206 +796,188 @@ __global__ __launch_bounds__(THREADS, 4) void k_tail(
     const uint32_t rid_base = cd.rid_base;
     const SortedView sv = sorted_view(&cs[c], cd, values, keysA, ridsA, keysB, ridsB, litsA,
                                       litsB);
-    uint32_t* out_keys = keys_out + cd.off;
-    uint32_t* out_offsets = offsets_out + cd.off;
-    uint32_t* out_words = words_out + 2 * cd.off;
-    // stage pairs [base - 1, base + TILE + HALO - 1) with 16-byte loads
-    const bool aligned = (((uintptr_t)(sv.keys + base) | (uintptr_t)(sv.rids + base) |
-                           (RUN ? (uintptr_t)(sv.lits + base) : (uintptr_t)0)) % 16) == 0;
+    const uint32_t j0 = base + (uint32_t)t * ITEMS;  // the lane's first item
+    const uint32_t nv = n > j0 ? (n - j0 < (uint32_t)ITEMS ? n - j0 : (uint32_t)ITEMS) : 0u;
+    const bool aligned = (((uintptr_t)sv.keys | (uintptr_t)sv.rids |
+                           (RUN ? (uintptr_t)sv.lits : (uintptr_t)0)) % 16) == 0;
+
+    // ---- items of the lane: key, chid, literal bits (0 past the column)
+    uint32_t key[ITEMS], ch[ITEMS], b[ITEMS];
     if (base + TILE <= n && aligned) {
-      const uint4* k4 = reinterpret_cast<const uint4*>(sv.keys + base);
-      const uint4* r4 = reinterpret_cast<const uint4*>(sv.rids + base);
-      for (int j = t; j < TILE / 4; j += THREADS) {
-        reinterpret_cast<uint4*>(s_k + PAD)[j] = k4[j];
-        uint4 rr;
+#pragma unroll
+      for (int q = 0; q < ITEMS / 4; ++q) {
+        const uint4 k4 = reinterpret_cast<const uint4*>(sv.keys + j0)[q];
+        key[4 * q] = k4.x; key[4 * q + 1] = k4.y; key[4 * q + 2] = k4.z; key[4 * q + 3] = k4.w;
         if (sv.implicit) {
-          const uint32_t r0 = rid_base + (uint32_t)(base + 4 * (uint64_t)j);
-          rr = make_uint4(r0, r0 + 1, r0 + 2, r0 + 3);
+          const uint32_t r0 = rid_base + j0 + 4u * q;
+          ch[4 * q] = r0; ch[4 * q + 1] = r0 + 1; ch[4 * q + 2] = r0 + 2; ch[4 * q + 3] = r0 + 3;
         } else {
-          rr = r4[j];
+          const uint4 r4 = reinterpret_cast<const uint4*>(sv.rids + j0)[q];
+          ch[4 * q] = r4.x; ch[4 * q + 1] = r4.y; ch[4 * q + 2] = r4.z; ch[4 * q + 3] = r4.w;
         }
-        reinterpret_cast<uint4*>(s_r + PAD)[j] = rr;
-        if (RUN) reinterpret_cast<uint4*>(s_l + PAD)[j] = reinterpret_cast<const uint4*>(sv.lits + base)[j];
+        if (RUN) {
+          const uint4 l4 = reinterpret_cast<const uint4*>(sv.lits + j0)[q];
+          b[4 * q] = l4.x; b[4 * q + 1] = l4.y; b[4 * q + 2] = l4.z; b[4 * q + 3] = l4.w;
+        }
       }
     } else {
-      for (int j = t; j < TILE; j += THREADS) {
-        const uint64_t g = base + j;
-        if (g < n) {
-          s_k[PAD + j] = sv.keys[g];
-          s_r[PAD + j] = sv.implicit ? rid_base + (uint32_t)g : sv.rids[g];
-          if (RUN) s_l[PAD + j] = sv.lits[g];
-        }
+#pragma unroll
+      for (int i = 0; i < ITEMS; ++i) {
+        const bool v = (uint32_t)i < nv;
+        const uint32_t g = j0 + i;
+        key[i] = v ? sv.keys[g] : 0u;
+        ch[i] = v ? (sv.implicit ? rid_base + g : sv.rids[g]) : 0u;
+        if (RUN) b[i] = v ? sv.lits[g] : 0u;
       }
     }
-    if (t < HALO) {
-      const uint64_t g = base + TILE + t;
-      if (g < n) {
-        s_k[PAD + TILE + t] = sv.keys[g];
-        s_r[PAD + TILE + t] = sv.implicit ? rid_base + (uint32_t)g : sv.rids[g];
-        if (RUN) s_l[PAD + TILE + t] = sv.lits[g];
+#pragma unroll
+    for (int i = 0; i < ITEMS; ++i) {  // ch[] held rids so far
+      const uint32_t r = ch[i];
+      ch[i] = r / 31u;
+      if (!RUN) b[i] = (uint32_t)i < nv ? 1u << (r - ch[i] * 31u) : 0u;
+    }
+    // predecessor of the lane's first item: lane l-1's last, or for lane 0
+    // the item before the wave (one load per wave)
+    uint32_t pk, pc;
+    {
+      const uint32_t gw = base + (uint32_t)(w * kWave * ITEMS);
+      uint32_t k0 = 0, c0 = 0;
+      if (gw > 0 && gw <= n) {
+        k0 = sv.keys[gw - 1];
+        c0 = (sv.implicit ? rid_base + (gw - 1) : sv.rids[gw - 1]) / 31u;
       }
+      pk = (uint32_t)__builtin_amdgcn_update_dpp((int)k0, (int)key[ITEMS - 1], 0x138, 0xF, 0xF, false);
+      pc = (uint32_t)__builtin_amdgcn_update_dpp((int)c0, (int)ch[ITEMS - 1], 0x138, 0xF, 0xF, false);
     }
-    if (t == THREADS - 1 && base > 0) {
-      s_k[PAD - 1] = sv.keys[base - 1];
-      s_r[PAD - 1] = sv.implicit ? rid_base + (uint32_t)(base - 1) : sv.rids[base - 1];
-    }
-    __syncthreads();
-
-    // Pass 1 over the wave's items: entry and key starts, fills, word and
-    // key counts.  bit[i] keeps the item's literal bits (1 << rid % 31 for a
-    // pair; 0 for none) for the literals, which pass 2 forms with prefix sums.
-    uint32_t fill[ITEMS], lit[ITEMS], key[ITEMS], ex[ITEMS], bit[ITEMS];
-    uint32_t khmask = 0;
-    uint32_t carry = 0;  // (keys << 16 | words) of this wave's earlier rows
+    // ---- entry and key starts, fills, counts
+    const bool colfirst = base == 0 && t == 0;
+    uint32_t f[ITEMS];
+    uint32_t stm = 0, khm = 0, fm = 0;  // per item: entry start, key start, nonzero fill
 #pragma unroll
     for (int i = 0; i < ITEMS; ++i) {
-      const int j = w * SUB + i * kWave + l;
-      const uint64_t g = base + j;
-      const int s = PAD + j;
-      fill[i] = 0;
-      lit[i] = 0;
-      key[i] = 0;
-      bit[i] = 0;
-      uint32_t p = 0;
-      // the previous item is lane l - 1 (wave shift right by one lane);
-      // lane 0 reads it from LDS
-      const uint32_t v = s_k[s], r = s_r[s], c = r / 31u;
-      uint32_t vprev = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x138, 0xF, 0xF, false);
-      uint32_t cprev = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)c, 0x138, 0xF, 0xF, false);
-      if (l == 0) {
-        vprev = s_k[s - 1];
-        cprev = s_r[s - 1] / 31u;
-      }
-      if (g < n) {
-        const bool first = g == 0;
-        const uint32_t cp = first ? 0u : cprev;
-        const bool kh = first || v != vprev;
-        bit[i] = RUN ? s_l[s] : 1u << (r % 31u);
-        if (kh || c != cp) {
-          lit[i] = 0x80000000u;  // entry start; payload bits added in pass 2
-          fill[i] = kh ? c : c - cp - 1u;
-          p = 1u + (fill[i] != 0u);
-          ++entries;
-        }
-        if (kh) {
-          key[i] = v;
-          khmask |= 1u << i;
-          p |= 1u << 16;
-        }
-      }
-      const uint32_t inc = wave_incl_add(p);
-      ex[i] = carry + inc - p;
-      carry += readlane(inc, kWave - 1);
+      const uint32_t kp = i ? key[i - 1] : pk, cp = i ? ch[i - 1] : pc;
+      const bool v = (uint32_t)i < nv;
+      const bool kh = v && (key[i] != kp || (i == 0 && colfirst));
+      const bool st = kh || (v && ch[i] != cp);
+      f[i] = st ? (kh ? ch[i] : ch[i] - cp - 1u) : 0u;
+      stm |= st ? 1u << i : 0u;
+      khm |= kh ? 1u << i : 0u;
+      fm |= f[i] != 0u ? 1u << i : 0u;
     }
-    if (l == 0) s_wtot[w] = carry;
-    // Literal bits of the entry run that continues into this wave's first
-    // 64 pairs from the previous wave (and, for the last wave, into the
-    // pairs past the tile): the bit sum of the pairs before the first entry
-    // start.  A run has at most 31 pairs (one chunk), so it ends inside the
-    // next 64.
+    const uint32_t nwords = (uint32_t)(__builtin_popcount(stm) + __builtin_popcount(fm));
+    entries += (uint32_t)__builtin_popcount(stm);
+    // lead bits: items before the lane's first stop (entry start or end of items)
+    const uint32_t stops = stm | (~0u << nv & kAll);
+    const uint32_t fs = stops ? (uint32_t)__builtin_ctz(stops) : (uint32_t)ITEMS;
+    uint32_t lead = 0;
+#pragma unroll
+    for (int i = 0; i < ITEMS; ++i) lead |= (uint32_t)i < fs ? b[i] : 0u;
+    // continuation of the lane's last run inside the wave: leads of lanes
+    // l+1 .. the next lane with a stop (at most 4 lanes on)
+    const uint64_t hsm = ballot(stops != 0u);
+    const uint64_t above = l == kWave - 1 ? 0ull : hsm >> (l + 1);
+    const uint32_t dist = above ? (uint32_t)__builtin_ctzll(above) + 1u : 64u;
+    uint32_t cont = 0;
     {
-      uint32_t b0 = bit[0];
-      bool h0 = lit[0] != 0u;
-      uint32_t lo, hi;
-      run_lead(b0, h0, &lo, &hi);
-      if (l == 0) {
-        s_lead[2 * w] = lo;
-        s_lead[2 * w + 1] = hi;
-      }
-      if (w == WAVES - 1) {
-        const int j = TILE + l;
-        const uint64_t g = base + j;
-        uint32_t bh = 0;
-        bool hh = false;
-        if (l < HALO && g < n) {
-          const int sh = PAD + j;
-          const uint32_t r = s_r[sh];
-          bh = RUN ? s_l[sh] : 1u << (r % 31u);
-          hh = s_k[sh] != s_k[sh - 1] || r / 31u != s_r[sh - 1] / 31u;
-        } else {
-          hh = true;  // nothing continues past the column
-        }
-        run_lead(bh, hh, &lo, &hi);
-        if (l == 0) {
-          s_lead[2 * WAVES] = lo;
-          s_lead[2 * WAVES + 1] = hi;
-        }
+      uint32_t sh = lead;
+#pragma unroll
+      for (uint32_t s = 1; s <= 4; ++s) {
+        sh = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)sh, 0x130, 0xF, 0xF, false);  // wave_shl:1
+        cont |= s <= dist ? sh : 0u;
       }
     }
+    // wave lead: lane 0's items before its first stop and, without one, the
+    // lanes after it
+    {
+      const uint32_t wl = readlane(stops ? lead : lead | cont, 0);
+      if (l == 0) s_wlead[w] = wl;
+    }
+    // next tile's lead: the run that crosses the tile end (last wave, 32 items)
+    if (w == WAVES - 1) {
+      const uint32_t g = base + (uint32_t)TILE + l;
+      const bool v = l < 32 && g < n;
+      uint32_t kk = 0, cc = 0, bb = 0;
+      if (v) {
+        const uint32_t r = sv.implicit ? rid_base + g : sv.rids[g];
+        kk = sv.keys[g];
+        cc = r / 31u;
+        bb = RUN ? sv.lits[g] : 1u << (r - cc * 31u);
+      }
+      const uint32_t kl = readlane(key[ITEMS - 1], kWave - 1), cl = readlane(ch[ITEMS - 1], kWave - 1);
+      const uint32_t kq = (uint32_t)__builtin_amdgcn_update_dpp((int)kl, (int)kk, 0x138, 0xF, 0xF, false);
+      const uint32_t cq = (uint32_t)__builtin_amdgcn_update_dpp((int)cl, (int)cc, 0x138, 0xF, 0xF, false);
+      const uint64_t sm = ballot(!v || kk != kq || cc != cq);  // lanes 32.. always stop
+      const int fh = __builtin_ctzll(sm);
+      const uint32_t tl = wave_or(l < fh ? bb : 0u);
+      if (l == 0) s_wlead[WAVES] = tl;
+    }
+    // tile-local word and key offsets: one wave scan of (keys << 16 | words)
+    const uint32_t packed = ((uint32_t)__builtin_popcount(khm) << 16) | nwords;
+    const uint32_t inc = wave_incl_add(packed);
+    if (l == kWave - 1) s_wtot[w] = inc;
     __syncthreads();
-    // Pass 2: literal of each entry start = OR of the bits of its run.  The
-    // bits of a run are distinct, so the OR is a sum: difference of wave
-    // prefix sums (low and high 16 bit positions summed apart) from the start
-    // to the lane before the next start, plus the continuation into the next
-    // 64 pairs when the run reaches lane 63.
+
+    // ---- literals: reverse sweep along the lane, seeded with the bits the
+    // lane's last run picks up from the lanes (wave, tile) after it
+    if (above == 0) cont |= s_wlead[w + 1];
+    if (nv < (uint32_t)ITEMS) cont = 0;  // the run ends with the items
+    uint32_t woff, tot;
     {
-      uint32_t nlo = s_lead[2 * (w + 1)], nhi = s_lead[2 * (w + 1) + 1];
-#pragma unroll
-      for (int i = ITEMS - 1; i >= 0; --i) {
-        const uint64_t hm = ballot(lit[i] != 0u);
-        if (hm == ballot(bit[i] != 0u)) {
-          // every item of these 64 starts its own entry (single rows): the
-          // literal is the item's own bits, plus, for lane 63, the run that
-          // continues into the next 64
-          if (lit[i]) lit[i] = 0x80000000u | bit[i] | (l == kWave - 1 ? (nhi << 16) | nlo : 0u);
-          nlo = nhi = 0;  // lane 0 starts an entry or holds no item
-          continue;
-        }
-        const uint32_t blo = bit[i] & 0xFFFFu, bhi = bit[i] >> 16;
-        const uint32_t slo = wave_incl_add(blo), shi = wave_incl_add(bhi);
-        const uint64_t rest = l == kWave - 1 ? 0ull : (hm >> (l + 1));
-        const int nh = rest ? l + 1 + __builtin_ctzll(rest) : kWave;
-        const uint32_t elo = (uint32_t)__shfl((int)slo, nh - 1, kWave);
-        const uint32_t ehi = (uint32_t)__shfl((int)shi, nh - 1, kWave);
-        uint32_t lo = elo - (slo - blo), hi = ehi - (shi - bhi);
-        if (nh == kWave) {
-          lo += nlo;
-          hi += nhi;
-        }
-        if (lit[i]) lit[i] = 0x80000000u | (hi << 16) | lo;
-        // this item's leading continuation, for item i - 1
-        const int fh = hm ? __builtin_ctzll(hm) : kWave;
-        const bool cont = (hm & 1ull) == 0;
-        nlo = cont ? (uint32_t)__builtin_amdgcn_readlane((int)slo, fh - 1) : 0u;
-        nhi = cont ? (uint32_t)__builtin_amdgcn_readlane((int)shi, fh - 1) : 0u;
-      }
+      // wave totals: one LDS read per lane and a wave scan
+      const uint32_t x = l < WAVES ? s_wtot[l] : 0u;
+      const uint32_t xi = wave_incl_add(x);
+      woff = readlane(xi - x, __builtin_amdgcn_readfirstlane(w));
+      tot = readlane(xi, WAVES - 1);
     }
-    uint32_t woff = 0, tot = 0;
-#pragma unroll
-    for (int i = 0; i < WAVES; ++i) {
-      woff += i < w ? s_wtot[i] : 0u;
-      tot += s_wtot[i];
-    }
-    const uint64_t tile_agg = ((uint64_t)(tot >> 16) << 31) | (tot & 0xFFFFu);
+    const uint32_t lpos = woff + inc - packed;  // (keys << 16 | words) before the lane
     if (w == 0) {
+      const uint64_t tile_agg = ((uint64_t)(tot >> 16) << 31) | (tot & 0xFFFFu);
       const uint64_t pre = lookback_wave_sum(status, tile, first_tile, tile_agg, fault);
       if (l == 0) s_prefix = pre;
     }
-    __syncthreads();  // every read of the staged pairs is done
-    const uint64_t pre = s_prefix;
-    const uint32_t Wpre = (uint32_t)(pre & 0x7FFFFFFFu), Kpre = (uint32_t)(pre >> 31);
+    {
+      // from the lane's last word back: [fill,] literal of each entry start
+      uint32_t carry = cont;
+      uint32_t pos = (lpos & 0xFFFFu) + nwords;
 #pragma unroll
-    for (int i = 0; i < ITEMS; ++i) {
-      const uint32_t e = woff + ex[i];
-      uint32_t pos = e & 0xFFFFu;
-      if ((khmask >> i) & 1u) {
-        const uint64_t kidx = (uint64_t)Kpre + (e >> 16);
-        if (kidx < n) {
-          out_keys[kidx] = key[i];
-          out_offsets[kidx] = Wpre + pos;
-        } else {
-          atomicOr(fault, 2u);
-        }
-      }
-      if (lit[i]) {
-        if (fill[i]) s_words[pos++] = fill[i];
-        s_words[pos] = lit[i];
+      for (int i = ITEMS - 1; i >= 0; --i) {
+        const bool st = (stm >> i) & 1u, fw = (fm >> i) & 1u;
+        carry |= b[i];
+        pos -= st ? 1u : 0u;
+        if (st) s_words[pos] = 0x80000000u | carry;
+        pos -= fw ? 1u : 0u;
+        if (fw) s_words[pos] = f[i];
+        carry = st ? 0u : carry;
       }
     }
-    __syncthreads();
+    __syncthreads();  // words staged, prefix published
+    const uint64_t pre = s_prefix;
+    const uint32_t Wpre = (uint32_t)(pre & 0x7FFFFFFFu), Kpre = (uint32_t)(pre >> 31);
+    uint32_t* out_keys = keys_out + cd.off;
+    uint32_t* out_offsets = offsets_out + cd.off;
+    uint32_t* out_words = words_out + 2 * cd.off;
+    if (khm) {
+      // key starts: the value (read again, the lines are still in L2) and
+      // the position of the key's first word
+#pragma unroll
+      for (int i = 0; i < ITEMS; ++i) {
+        if ((khm >> i) & 1u) {
+          const uint32_t below = (1u << i) - 1u;
+          const uint32_t kg = Kpre + (lpos >> 16) + __builtin_popcount(khm & below);
+          const uint32_t wp = (lpos & 0xFFFFu) + __builtin_popcount(stm & below) +
+                              __builtin_popcount(fm & below);
+          if (kg < n) {
+            out_keys[kg] = sv.keys[j0 + i];
+            out_offsets[kg] = Wpre + wp;
+          } else {
+            atomicOr(fault, 2u);
+          }
+        }
+      }
+    }
     const uint32_t Wt = tot & 0xFFFFu;
     for (uint32_t j = t; j < Wt; j += THREADS) {
-      const uint64_t o = (uint64_t)Wpre + j;
+      const uint32_t o = Wpre + j;
       if (o < 2 * n) out_words[o] = s_words[j];
       else atomicOr(fault, 2u);
     }
@@ -1017,7 +985,8 @@ __global__ __launch_bounds__(THREADS, 4) void k_tail(
       counts[cd.index].length = (uint64_t)Wpre + Wt;
       counts[cd.index].keycnt = (uint64_t)Kpre + (tot >> 16);
     }
-    __syncthreads();
+    // the next tile's first barrier (in next_tile) orders these LDS reads
+    // before its writes
   }
   if (ent_col != 0xFFFFFFFFu) flush_entries();
 }
@@ -1338,11 +1307,24 @@ static uint32_t cap_grid(uint32_t work, uint32_t cap) {
   return g ? g : 1u;
 }
 
+// Radix blocks resident per CU: the waves the register budget allows
+// (WAH_SORT_WAVES_PER_EU per SIMD) and the LDS of k_onesweep (staged keys
+// and digits, per-wave digit counts, digit tables; 160 KiB per CU).
+template <int T, int I>
+constexpr uint32_t sort_blocks_per_cu() {
+  constexpr uint32_t by_waves = WAH_SORT_WAVES_PER_EU * 4 / (T / kWave);
+  constexpr uint32_t lds = T * I * 5 + (T / kWave) * 1024 + 2 * 1024 + 64;
+  constexpr uint32_t by_lds = 160u * 1024u / lds;
+  constexpr uint32_t b = by_waves < by_lds ? by_waves : by_lds;
+  return b ? b : 1u;
+}
+
 // The run cut and the four radix launches for one tile shape.
 template <int T, int I>
 static void launch_radix(const BuildArgs& a, hipStream_t s, LaunchHook* hook) {
-  const uint32_t grid =
-      cap_grid(a.sort_tiles_cap, WAH_STATIC_TILES ? a.sort_tiles_cap : a.max_resident * 256 / T);
+  const uint32_t grid = cap_grid(a.sort_tiles_cap, WAH_STATIC_TILES
+                                                       ? a.sort_tiles_cap
+                                                       : a.max_resident / 4 * sort_blocks_per_cu<T, I>());
   // the cut needs half the registers of a radix pass: two blocks per CU
   const uint32_t cut_grid = cap_grid(a.sort_tiles_cap, 2 * grid);
   hipLaunchKernelGGL((k_runcut<T, I>), dim3(cut_grid), dim3(T), 0, s, a.values, a.keysB, a.ridsB,
@@ -1390,15 +1372,16 @@ void launch_sort(const BuildArgs& a, hipStream_t s, LaunchHook* hook) {
 void launch_tail(const BuildArgs& a, hipStream_t s, LaunchHook* hook) {
   const uint32_t grid =
       cap_grid(a.tail_tiles_cap, WAH_STATIC_TILES ? a.tail_tiles_cap
-                                                  : a.max_resident * 256 / kTailThreads);
+                                                  : a.max_resident / 4 * kTailBlocksPerCU);
+  const uint32_t lanes = WAH_TAIL_LANES ? WAH_TAIL_LANES : tile_lanes(grid);
   hipLaunchKernelGGL((k_tail<kTailThreads, kTailItems, false>), dim3(grid), dim3(kTailThreads),
                      0, s, a.desc, a.cs, a.hdr, a.values, a.keysA, a.ridsA, a.keysB, a.ridsB,
                      a.litsA, a.litsB, a.out_keys, a.out_offsets, a.out_words, a.tail_status,
-                     tile_lanes(grid), a.counts);
+                     lanes, a.counts);
   hipLaunchKernelGGL((k_tail<kTailThreads, kTailItems, true>), dim3(grid), dim3(kTailThreads),
                      0, s, a.desc, a.cs, a.hdr, a.values, a.keysA, a.ridsA, a.keysB, a.ridsB,
                      a.litsA, a.litsB, a.out_keys, a.out_offsets, a.out_words, a.tail_status,
-                     tile_lanes(grid), a.counts);
+                     lanes, a.counts);
   if (hook) hook->mark(s);
   uint64_t lg = (a.n_cap + kBlock - 1) / kBlock;
   hipLaunchKernelGGL(k_finish, dim3(cap_grid(lg > 1024 ? 1024u : (uint32_t)lg, 1024)),

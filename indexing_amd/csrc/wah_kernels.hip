@@ -267,13 +267,29 @@ __global__ __launch_bounds__(kBlock) void k_plan(const DescHead* __restrict__ de
       const Column cd = desc_cols(desc)[c];
       uint32_t* tr = tile_runs + cd.sort_tile0;
       const uint32_t tiles = sort_tiles_of(n, sort_tile);
-      uint64_t carry = 0;
-      for (uint32_t b = 0; b < tiles; b += kBlock) {
-        const uint32_t x = b + t < tiles ? tr[b + t] : 0u;
-        uint64_t tot;
-        const uint64_t ex = block_exclusive<OpAdd>(x, s_scan, &tot);
-        if (b + t < tiles) tr[b + t] = (uint32_t)(carry + ex);
-        carry += tot;
+      // thread t scans a strip of consecutive tiles; loads in batches of 16
+      const uint32_t per = (tiles + kBlock - 1) / kBlock;
+      const uint32_t s0 = t * per < tiles ? t * per : tiles;
+      const uint32_t s1 = s0 + per < tiles ? s0 + per : tiles;
+      uint64_t sum = 0;
+      for (uint32_t b = s0; b < s1; b += 16) {
+        uint32_t x[16];
+#pragma unroll
+        for (int i = 0; i < 16; ++i) x[i] = b + i < s1 ? tr[b + i] : 0u;
+#pragma unroll
+        for (int i = 0; i < 16; ++i) sum += x[i];
+      }
+      uint64_t tot;
+      uint64_t run = block_exclusive<OpAdd>(sum, s_scan, &tot);
+      for (uint32_t b = s0; b < s1; b += 16) {
+        uint32_t x[16];
+#pragma unroll
+        for (int i = 0; i < 16; ++i) x[i] = b + i < s1 ? tr[b + i] : 0u;
+#pragma unroll
+        for (int i = 0; i < 16; ++i) {
+          if (b + i < s1) tr[b + i] = (uint32_t)run;
+          run += x[i];
+        }
       }
     }
     for (int p = 0; p < 4; ++p) {
@@ -315,127 +331,155 @@ __global__ __launch_bounds__(kBlock) void k_plan(const DescHead* __restrict__ de
 // (stability).  Keys and then rids are staged in one LDS buffer in digit
 // order and written out in runs per digit.
 // ---------------------------------------------------------------------------
-// Run-mode helpers for launch 0, which reads the rows in order: a row starts
-// a run unless it repeats its predecessor's value inside the same chunk.
-// stop = lanes that start a run or hold no row; a run's length is the
-// distance to the next stop, continuing into the next 64 rows (lead = their
-// leading non-stop lanes) when it reaches lane 63.
-__device__ __forceinline__ uint32_t run_length(uint64_t stop, uint32_t l, uint32_t lead) {
-  const uint64_t rest = l == kWave - 1 ? 0ull : (stop >> (l + 1));
-  return rest ? 1u + (uint32_t)__builtin_ctzll(rest) : (uint32_t)kWave - l + lead;
-}
-
+// ---------------------------------------------------------------------------
 // Run mode, before the radix launches: cuts each run-mode column into runs
 // (a row starts one unless it repeats its predecessor's value inside the
-// same chunk) and writes one item per run, in row order, to buffer B:
+// same 31-row chunk) and writes one item per run, in row order, to buffer B:
 // value, rid of its first row, literal bits of its rows.  An ordered stream
-// compaction without look-back: the histogram kernel counted the runs of
-// each radix tile and k_plan turned the counts into per-tile offsets; per
-// tile, ballots and a block scan over the waves place each run.
-template <int THREADS, int ITEMS>
-__global__ __launch_bounds__(THREADS) void k_runcut(
+// compaction without look-back: the histogram counted the runs of each
+// radix tile and k_plan turned the counts into per-tile offsets.  A block
+// takes whole radix tiles (static grid stride) and walks each in steps of
+// kBlock x 16 rows with a running offset.  Lane t owns 16 consecutive rows
+// (16-byte loads); a run is at most 31 rows, so the run of a lane's last
+// start ends within the next two lanes (or the next wave's first two, or
+// the rows past the step, exchanged in LDS).  Items are staged in LDS and
+// stored contiguously, one array at a time.
+// ---------------------------------------------------------------------------
+constexpr int kCutRows = 16;                    // rows per lane
+constexpr int kCutStep = kBlock * kCutRows;     // rows per block step
+__global__ __launch_bounds__(kBlock, 8) void k_runcut(
     const uint32_t* __restrict__ values, uint32_t* __restrict__ keysB,
     uint32_t* __restrict__ ridsB, uint32_t* __restrict__ litsB,
     const DescHead* __restrict__ desc, const ColState* __restrict__ cs,
-    Header* __restrict__ hdr, const uint32_t* __restrict__ tile_runs, uint32_t lanes) {
-  constexpr int TILE = THREADS * ITEMS;
-  constexpr int WAVES = THREADS / kWave;
-  __shared__ uint32_t s_lead[WAVES + 1];  // leading non-stop lanes of each wave's first 64 rows
-  __shared__ uint32_t s_wsum[WAVES];
-  __shared__ uint32_t s_tile;
+    Header* __restrict__ hdr, const uint32_t* __restrict__ tile_runs, uint32_t sort_tile) {
+  constexpr int R = kCutRows;
+  constexpr uint32_t kAll = (1u << R) - 1u;
+  __shared__ uint32_t s_stage[kCutStep];
+  __shared__ uint32_t s_wsum[kWaves];
+  __shared__ uint32_t s_wlead[kWaves + 1];  // rows before the first stop of wave w; [kWaves]: past the step
   const Dims dims = desc->dims;
   const Column* cols = desc_cols(desc);
   const int t = threadIdx.x, w = t / kWave, l = t % kWave;
   if (hdr->mode_cols[1] == 0) return;  // no run-mode column
-  uint32_t iter = 0;
-  for (;;) {
-    const uint32_t tile = __builtin_amdgcn_readfirstlane(
-        next_tile(hdr->tiles[kCounterCut], lanes, &s_tile, iter));
-    if (tile >= dims.sort_tiles) break;
+  for (uint32_t tile = blockIdx.x; tile < dims.sort_tiles; tile += gridDim.x) {
     const uint32_t c = find_segment(&cols[0].sort_tile0, sizeof(Column) / 4, dims.ncols, tile);
     const ColState* col = &cs[c];
-    if (col->mode != kModeRun) {
-      __syncthreads();  // s_tile is read by every wave before it is rewritten
-      continue;
-    }
+    if (col->mode != kModeRun) continue;  // block-uniform, no LDS touched
     const Column cd = cols[c];
-    const uint64_t n = cd.n;
-    const uint64_t base = (uint64_t)(tile - cd.sort_tile0) * TILE;
-    const uint64_t wbase = base + (uint64_t)w * kWave * ITEMS + l;
+    // column-local rows fit 32 bits (a column holds fewer than 2^30)
+    const uint32_t n = (uint32_t)cd.n;
+    const uint32_t t0 = (tile - cd.sort_tile0) * sort_tile;
+    const uint32_t t1 = t0 + sort_tile < n ? t0 + sort_tile : n;
     const uint32_t* vp = values + cd.off;
-    uint32_t key[ITEMS];
-    uint64_t startm[ITEMS], stop[ITEMS];
+    const bool aligned = ((uintptr_t)vp % 16) == 0;
+    const uint32_t items = col->items;
+    uint32_t off = tile_runs[tile];  // runs of the column's earlier rows
+    for (uint32_t lo = t0; lo < t1; lo += kCutStep) {
+      const uint32_t j0 = lo + (uint32_t)t * R;  // the lane's first row
+      const uint32_t nv = t1 > j0 ? (t1 - j0 < (uint32_t)R ? t1 - j0 : (uint32_t)R) : 0u;
+      uint32_t v[R];
+      if (lo + kCutStep <= t1 && aligned) {
 #pragma unroll
-    for (int i = 0; i < ITEMS; ++i) {
-      const uint64_t pos = wbase + (uint64_t)(i * kWave);
-      key[i] = pos < n ? vp[pos] : 0u;
-    }
-    uint32_t mine = 0;  // run starts of this wave
-    uint32_t m = (cd.rid_base + (uint32_t)wbase) % 31u;  // row % 31 of item i (rows 64 apart)
+        for (int q = 0; q < R / 4; ++q) {
+          const uint4 x = reinterpret_cast<const uint4*>(vp + j0)[q];
+          v[4 * q] = x.x; v[4 * q + 1] = x.y; v[4 * q + 2] = x.z; v[4 * q + 3] = x.w;
+        }
+      } else {
 #pragma unroll
-    for (int i = 0; i < ITEMS; ++i) {
-      const uint64_t pos = wbase + (uint64_t)(i * kWave);
-      const bool valid = pos < n;
-      // previous row: lane l - 1 (wave shift by one lane), lane 0 of item i
-      // from lane 63 of item i - 1, lane 0 of item 0 from memory
-      uint32_t prev = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)key[i], 0x138, 0xF, 0xF, false);
-      if (i > 0) {
-        const uint32_t last = readlane(key[i > 0 ? i - 1 : 0], kWave - 1);
-        if (l == 0) prev = last;
-      } else if (l == 0 && valid && pos > 0) {
-        prev = vp[pos - 1];
+        for (int i = 0; i < R; ++i) v[i] = (uint32_t)i < nv ? vp[j0 + i] : 0u;
       }
-      const bool start = valid && (pos == 0 || m == 0u || key[i] != prev);
-      m += 2u;  // 64 rows on: 64 % 31 = 2
-      m = m >= 31u ? m - 31u : m;
-      startm[i] = ballot(start);
-      stop[i] = ballot(start || !valid);
-      mine += (uint32_t)__popcll(startm[i]);
-    }
-    if (l == 0) {
-      s_lead[w] = (uint32_t)__builtin_ctzll(stop[0]);
-      s_wsum[w] = mine;
-    }
-    if (w == WAVES - 1) {  // the rows past the tile: where its last run ends
-      const uint64_t pos = base + TILE + l;
-      const bool valid = pos < n;
-      const uint32_t v = valid ? vp[pos] : 0u;
-      uint32_t prev = (uint32_t)__shfl_up((int)v, 1, kWave);
-      const uint32_t last = readlane(key[ITEMS - 1], kWave - 1);
-      if (l == 0) prev = last;
-      const bool start = valid && ((cd.rid_base + (uint32_t)pos) % 31u == 0u || v != prev);
-      const uint64_t m = ballot(start || !valid);
-      if (l == 0) s_lead[WAVES] = (uint32_t)__builtin_ctzll(m);
-    }
-    __syncthreads();
-    uint32_t before = 0, total = 0;
+      // predecessor of the lane's first row: lane l-1's last, or for lane 0
+      // the row before the wave (one load per wave)
+      uint32_t pv;
+      {
+        const uint32_t gw = lo + (uint32_t)(w * kWave * R);
+        const uint32_t p0 = gw > 0 && gw <= n ? vp[gw - 1] : 0u;
+        pv = (uint32_t)__builtin_amdgcn_update_dpp((int)p0, (int)v[R - 1], 0x138, 0xF, 0xF, false);
+      }
+      const uint32_t m0 = (cd.rid_base + j0) % 31u;  // row % 31 of the lane's first row
+      uint32_t sm = 0;  // run starts
+      {
+        uint32_t m = m0;
 #pragma unroll
-    for (int i = 0; i < WAVES; ++i) {
-      before += i < w ? s_wsum[i] : 0u;
-      total += s_wsum[i];
-    }
-    (void)total;
-    uint64_t idx = (uint64_t)tile_runs[tile] + before;  // runs of the column's earlier tiles
-    const uint64_t items = col->items;
-#pragma unroll
-    for (int i = 0; i < ITEMS; ++i) {
-      if ((startm[i] >> l) & 1ull) {
-        const uint32_t lead = i + 1 < ITEMS ? (uint32_t)__builtin_ctzll(stop[i + 1 < ITEMS ? i + 1 : i])
-                                            : s_lead[w + 1];
-        const uint32_t len = run_length(stop[i], (uint32_t)l, lead);
-        const uint32_t row = cd.rid_base + (uint32_t)(wbase + (uint64_t)(i * kWave));
-        const uint64_t j = idx + mbcnt(startm[i]);
-        if (j < items) {
-          keysB[cd.off + j] = key[i];
-          ridsB[cd.off + j] = row;
-          litsB[cd.off + j] = ((len >= 32u ? 0u : (1u << len)) - 1u) << (row % 31u);
-        } else {
-          atomicOr(&hdr->fault, 2u);
+        for (int i = 0; i < R; ++i) {
+          const uint32_t prev = i ? v[i - 1] : pv;
+          const bool st = (uint32_t)i < nv && (j0 + i == 0 || m == 0u || v[i] != prev);
+          sm |= st ? 1u << i : 0u;
+          m = m == 30u ? 0u : m + 1u;
         }
       }
-      idx += (uint64_t)__popcll(startm[i]);
+      const uint32_t stops = sm | (~0u << nv & kAll);
+      const uint32_t lead = stops ? (uint32_t)__builtin_ctz(stops) : (uint32_t)R;
+      const uint32_t cnt = (uint32_t)__builtin_popcount(sm);
+      const uint32_t inc = wave_incl_add(cnt);
+      if (l == kWave - 1) s_wsum[w] = inc;
+      const uint64_t hsm = ballot(stops != 0u);
+      const uint32_t L1 = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)lead, 0x130, 0xF, 0xF, false);
+      const uint32_t L2 = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)L1, 0x130, 0xF, 0xF, false);
+      {
+        // rows before the wave's first stop (within its first two lanes)
+        const uint32_t l0 = readlane(lead, 0), l1 = readlane(lead, 1);
+        if (l == 0) s_wlead[w] = (hsm & 1ull) ? l0 : (uint32_t)R + l1;
+      }
+      if (w == kWaves - 1) {  // rows past the step: where its last run ends
+        const uint32_t g = lo + (uint32_t)kCutStep + l;
+        const bool valid = l < 32 && g < n;
+        const uint32_t x = valid ? vp[g] : 0u;
+        const uint32_t last = readlane(v[R - 1], kWave - 1);
+        const uint32_t px = (uint32_t)__builtin_amdgcn_update_dpp((int)last, (int)x, 0x138, 0xF, 0xF, false);
+        const bool st = !valid || (cd.rid_base + g) % 31u == 0u || x != px;
+        const uint64_t sb = ballot(st);  // lanes 32.. always stop
+        if (l == 0) s_wlead[kWaves] = (uint32_t)__builtin_ctzll(sb);
+      }
+      __syncthreads();
+      uint32_t woff, tot;
+      {
+        const uint32_t x = l < kWaves ? s_wsum[l] : 0u;
+        const uint32_t xi = wave_incl_add(x);
+        woff = readlane(xi - x, __builtin_amdgcn_readfirstlane(w));
+        tot = readlane(xi, kWaves - 1);
+      }
+      // rows after the lane that the run of its last start covers
+      const uint32_t nw = s_wlead[w + 1];
+      uint32_t cont;
+      if (l == kWave - 1) cont = nw;
+      else if ((hsm >> (l + 1)) & 1ull) cont = L1;
+      else if (l == kWave - 2) cont = (uint32_t)R + nw;
+      else cont = (uint32_t)R + L2;
+      if (nv < (uint32_t)R) cont = 0;  // the run ends with the rows
+      const uint32_t k0 = woff + inc - cnt;  // items of the step before the lane
+      if (off + tot > items) {
+        if (t == 0) atomicOr(&hdr->fault, 2u);
+        return;  // block-uniform: tile_runs disagree with the items
+      }
+      // three rounds through LDS: values, rids, literal bits
+#pragma unroll 1
+      for (int round = 0; round < 3; ++round) {
+        uint32_t k = k0;
+#pragma unroll
+        for (int i = 0; i < R; ++i) {
+          if ((sm >> i) & 1u) {
+            uint32_t x;
+            if (round == 0) {
+              x = v[i];
+            } else if (round == 1) {
+              x = cd.rid_base + j0 + i;
+            } else {
+              const uint32_t rest = stops >> (i + 1);
+              const uint32_t len = rest ? (uint32_t)__builtin_ctz(rest) + 1u : (uint32_t)(R - i) + cont;
+              const uint32_t mi = m0 + i >= 31u ? m0 + i - 31u : m0 + i;
+              x = ((len >= 32u ? 0u : (1u << len)) - 1u) << mi;
+            }
+            s_stage[k++] = x;
+          }
+        }
+        __syncthreads();
+        uint32_t* dst = (round == 0 ? keysB : round == 1 ? ridsB : litsB) + cd.off + off;
+        for (uint32_t j = t; j < tot; j += kBlock) dst[j] = s_stage[j];
+        __syncthreads();
+      }
+      off += tot;
     }
-    __syncthreads();
   }
 }
 
@@ -1325,10 +1369,9 @@ static void launch_radix(const BuildArgs& a, hipStream_t s, LaunchHook* hook) {
   const uint32_t grid = cap_grid(a.sort_tiles_cap, WAH_STATIC_TILES
                                                        ? a.sort_tiles_cap
                                                        : a.max_resident / 4 * sort_blocks_per_cu<T, I>());
-  // the cut needs half the registers of a radix pass: two blocks per CU
-  const uint32_t cut_grid = cap_grid(a.sort_tiles_cap, 2 * grid);
-  hipLaunchKernelGGL((k_runcut<T, I>), dim3(cut_grid), dim3(T), 0, s, a.values, a.keysB, a.ridsB,
-                     a.litsB, a.desc, a.cs, a.hdr, a.tile_runs, tile_lanes(cut_grid));
+  hipLaunchKernelGGL(k_runcut, dim3(cap_grid(a.sort_tiles_cap, 2 * a.max_resident)), dim3(kBlock),
+                     0, s, a.values, a.keysB, a.ridsB, a.litsB, a.desc, a.cs, a.hdr, a.tile_runs,
+                     (uint32_t)(T * I));
   if (hook) hook->mark(s);
   // each launch runs the a-th pass of pair-mode columns, then of run-mode
   // columns; a kernel with no such column returns at once

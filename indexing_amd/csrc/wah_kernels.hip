@@ -587,154 +587,161 @@ __global__ __launch_bounds__(THREADS, WAH_SORT_WAVES_PER_EU) void k_onesweep(
     }
     const uint32_t shift = col->shift[a];
     if (t < 256) status_next[(uint64_t)tile * 256 + t] = 0;  // next launch's state
-
-    // Keys only for now: the rids are loaded once the keys are staged, so
-    // they never occupy registers during ranking and look-back.
-    uint32_t key[ITEMS], rank[ITEMS];
     const uint64_t wbase = base + (uint64_t)w * kWave * ITEMS + l;
-    const bool full = base + TILE <= n;
-    const uint32_t nvalid = full ? (uint32_t)ITEMS * kWave : (uint32_t)(n > wbase ? n - wbase : 0);
-    {
-      const uint32_t* kp = src_keys + cd.off + wbase;  // one base, immediate offsets
-      if (full) {
+    // A full tile runs a straight-line body (no per-item guards), so the
+    // compiler can wait for the rid loads without draining the key stores.
+    auto sweep = [&](auto full_tag) {
+      constexpr bool FULL = decltype(full_tag)::value;
+      // Keys only for now: the rids are loaded once the keys are staged, so
+      // they never occupy registers during ranking and look-back.
+      uint32_t key[ITEMS], rank[ITEMS];
+      const uint32_t nvalid =
+          FULL ? (uint32_t)ITEMS * kWave : (uint32_t)(n > wbase ? n - wbase : 0);
+      {
+        const uint32_t* kp = src_keys + cd.off + wbase;  // one base, immediate offsets
 #pragma unroll
-        for (int i = 0; i < ITEMS; ++i) key[i] = kp[i * kWave];
-      } else {
-#pragma unroll
-        for (int i = 0; i < ITEMS; ++i) key[i] = (uint32_t)(i * kWave) < nvalid ? kp[i * kWave] : 0u;
+        for (int i = 0; i < ITEMS; ++i)
+          key[i] = FULL || (uint32_t)(i * kWave) < nvalid ? kp[i * kWave] : 0u;
       }
-    }
-    uint32_t part = 0;  // items this lane ranks
-#pragma unroll
-    for (int i = 0; i < ITEMS; ++i) part |= ((uint32_t)(i * kWave) < nvalid ? 1u : 0u) << i;
-    // Rank: peers = lanes with the same digit (8 ballots); rank = peers
-    // below this lane + keys of that digit the wave ranked before.
-#pragma unroll
-    for (int i = 0; i < ITEMS; ++i) {
-      if ((part >> i) & 1u) {
-        const uint32_t d = (key[i] >> shift) & 255u;
-        if (RANK == kRankAtomic) {
-          rank[i] = atomicAdd(&s_wcnt[w][d], 1u);
-        } else {
-          uint64_t peers = ballot(true);
-#pragma unroll
-          for (int b = 0; b < 8; ++b) {
-            const bool bit = (d >> b) & 1u;
-            const uint64_t bb = ballot(bit);
-            peers &= bit ? bb : ~bb;
-          }
-          const uint32_t cnt = (uint32_t)__popcll(peers);
-          const uint32_t below = mbcnt(peers);
-          const uint32_t prior = s_wcnt[w][d];
-          rank[i] = prior + below;
-          if (below == cnt - 1) s_wcnt[w][d] = prior + cnt;
-        }
-      }
-    }
-    __syncthreads();
-    // Threads 0..255 own one digit each: wave prefixes, tile total,
-    // exclusive scan over digits, look-back.
-    uint32_t total = 0;
-    if (t < 256) {
-      const uint32_t goff = col->digit_offset[shift / 8][t];
-#pragma unroll
-      for (int i = 0; i < WAVES; ++i) {
-        const uint32_t x = s_wcnt[i][t];
-        s_wcnt[i][t] = total;
-        total += x;
-      }
-      uint32_t* my = &status[(uint64_t)tile * 256 + t];
-      st_agent(my, ((tile == first ? kFlagPrefix : kFlagAgg) << 30) | total);
-      const uint32_t inc = wave_incl_add(total);
-      if (l == kWave - 1) s_dsum[w] = inc;
-      s_excl[t] = inc - total;  // within this wave's 64 digits
-      uint32_t prev = 0;
-      if (tile > first) {
-        prev = lookback_digit(status + t, tile, first, 256, fault);
-        st_agent(my, (kFlagPrefix << 30) | (prev + total));
-      }
-      s_glob[t] = goff + prev;  // the tile-local offset is subtracted below
-    }
-    __syncthreads();
-    if (t < 256) {
-      uint32_t before = 0;
-#pragma unroll
-      for (int i = 0; i < 4; ++i) before += i < w ? s_dsum[i] : 0u;
-      const uint32_t ex = s_excl[t] + before;
-      s_excl[t] = ex;
-      s_glob[t] -= ex;  // column index = s_glob[d] + local index
-      if (t == 255) s_tilen = ex + total;
-    }
-    __syncthreads();
-#pragma unroll
-    for (int i = 0; i < ITEMS; ++i) {
-      if ((part >> i) & 1u) {
-        const uint32_t d = (key[i] >> shift) & 255u;
-        const uint32_t pos = s_excl[d] + s_wcnt[w][d] + rank[i];
-        s_stage[pos] = key[i];
-        s_digit[pos] = (uint8_t)d;
-        rank[i] = pos;  // keep the LDS slot for the rids
-      }
-    }
-    // LDS slots, two per register (a tile has at most 2^16 items)
-    uint32_t slot2[(ITEMS + 1) / 2];
-#pragma unroll
-    for (int i = 0; i < ITEMS; i += 2)
-      slot2[i / 2] = (rank[i] & 0xFFFFu) | (i + 1 < ITEMS ? rank[i + 1 < ITEMS ? i + 1 : i] << 16 : 0u);
-    // rids: issued now, in flight while the keys go out
-    uint32_t rid[ITEMS];
-    if (implicit) {
-#pragma unroll
-      for (int i = 0; i < ITEMS; ++i) rid[i] = cd.rid_base + (uint32_t)wbase + (uint32_t)(i * kWave);
-    } else {
-      const uint32_t* rp = src_rids + cd.off + wbase;
-#pragma unroll
-      for (int i = 0; i < ITEMS; ++i) rid[i] = (uint32_t)(i * kWave) < nvalid ? rp[i * kWave] : 0u;
-    }
-    __syncthreads();
-    const uint32_t tile_n = s_tilen;
-    uint32_t* dk = dst_keys + cd.off;
-    uint32_t* dr = dst_rids + cd.off;
-#pragma unroll
-    for (int j = 0; j < ITEMS; ++j) {
-      const uint32_t q = (uint32_t)j * THREADS + t;
-      if (q < tile_n) {
-        const uint32_t g = s_glob[s_digit[q]] + q;
-        if (g < n_out) dk[g] = s_stage[q];
-        else atomicOr(fault, 2u);
-      }
-    }
-    __syncthreads();
-#pragma unroll
-    for (int i = 0; i < ITEMS; ++i)
-      if ((part >> i) & 1u) s_stage[(slot2[i / 2] >> (16 * (i % 2))) & 0xFFFFu] = rid[i];
-    __syncthreads();
-#pragma unroll
-    for (int j = 0; j < ITEMS; ++j) {
-      const uint32_t q = (uint32_t)j * THREADS + t;
-      if (q < tile_n) {
-        const uint32_t g = s_glob[s_digit[q]] + q;
-        if (g < n_out) dr[g] = s_stage[q];
-      }
-    }
-    if (RUN) {
-      // literal bits of each run item, moved along with its key and rid
-      __syncthreads();
-      const uint32_t* lp = src_lits + cd.off + wbase;  // one base, immediate offsets
+      uint32_t part = 0;  // items this lane ranks
 #pragma unroll
       for (int i = 0; i < ITEMS; ++i)
-        if ((part >> i) & 1u) s_stage[(slot2[i / 2] >> (16 * (i % 2))) & 0xFFFFu] = lp[i * kWave];
-      __syncthreads();
-      uint32_t* dl = dst_lits + cd.off;
-#pragma unroll 4  // a full unroll keeps the slot indices of all three rounds live
-      for (int j = 0; j < ITEMS; ++j) {
-        const uint32_t q = (uint32_t)j * THREADS + t;
-        if (q < tile_n) {
-          const uint32_t g = s_glob[s_digit[q]] + q;
-          if (g < n_out) dl[g] = s_stage[q];
+        part |= (FULL || (uint32_t)(i * kWave) < nvalid ? 1u : 0u) << i;
+      // Rank: peers = lanes with the same digit (8 ballots); rank = peers
+      // below this lane + keys of that digit the wave ranked before.
+#pragma unroll
+      for (int i = 0; i < ITEMS; ++i) {
+        if (FULL || ((part >> i) & 1u)) {
+          const uint32_t d = (key[i] >> shift) & 255u;
+          if (RANK == kRankAtomic) {
+            rank[i] = atomicAdd(&s_wcnt[w][d], 1u);
+          } else {
+            uint64_t peers = ballot(true);
+#pragma unroll
+            for (int b = 0; b < 8; ++b) {
+              const bool bit = (d >> b) & 1u;
+              const uint64_t bb = ballot(bit);
+              peers &= bit ? bb : ~bb;
+            }
+            const uint32_t cnt = (uint32_t)__popcll(peers);
+            const uint32_t below = mbcnt(peers);
+            const uint32_t prior = s_wcnt[w][d];
+            rank[i] = prior + below;
+            if (below == cnt - 1) s_wcnt[w][d] = prior + cnt;
+          }
         }
       }
-    }
+      __syncthreads();
+      // Threads 0..255 own one digit each: wave prefixes, tile total,
+      // exclusive scan over digits, look-back.
+      uint32_t total = 0;
+      if (t < 256) {
+        const uint32_t goff = col->digit_offset[shift / 8][t];
+#pragma unroll
+        for (int i = 0; i < WAVES; ++i) {
+          const uint32_t x = s_wcnt[i][t];
+          s_wcnt[i][t] = total;
+          total += x;
+        }
+        uint32_t* my = &status[(uint64_t)tile * 256 + t];
+        st_agent(my, ((tile == first ? kFlagPrefix : kFlagAgg) << 30) | total);
+        const uint32_t inc = wave_incl_add(total);
+        if (l == kWave - 1) s_dsum[w] = inc;
+        s_excl[t] = inc - total;  // within this wave's 64 digits
+        uint32_t prev = 0;
+        if (tile > first) {
+          prev = lookback_digit(status + t, tile, first, 256, fault);
+          st_agent(my, (kFlagPrefix << 30) | (prev + total));
+        }
+        s_glob[t] = goff + prev;  // the tile-local offset is subtracted below
+      }
+      __syncthreads();
+      if (t < 256) {
+        uint32_t before = 0;
+#pragma unroll
+        for (int i = 0; i < 4; ++i) before += i < w ? s_dsum[i] : 0u;
+        const uint32_t ex = s_excl[t] + before;
+        s_excl[t] = ex;
+        s_glob[t] -= ex;  // column index = s_glob[d] + local index
+        if (t == 255) s_tilen = ex + total;
+      }
+      __syncthreads();
+#pragma unroll
+      for (int i = 0; i < ITEMS; ++i) {
+        if (FULL || ((part >> i) & 1u)) {
+          const uint32_t d = (key[i] >> shift) & 255u;
+          const uint32_t pos = s_excl[d] + s_wcnt[w][d] + rank[i];
+          s_stage[pos] = key[i];
+          s_digit[pos] = (uint8_t)d;
+          rank[i] = pos;  // keep the LDS slot for the rids
+        }
+      }
+      // LDS slots, two per register (a tile has at most 2^16 items)
+      uint32_t slot2[(ITEMS + 1) / 2];
+#pragma unroll
+      for (int i = 0; i < ITEMS; i += 2)
+        slot2[i / 2] = (rank[i] & 0xFFFFu) | (i + 1 < ITEMS ? rank[i + 1 < ITEMS ? i + 1 : i] << 16 : 0u);
+      // rids: issued now, in flight while the keys go out
+      uint32_t rid[ITEMS];
+      if (implicit) {
+#pragma unroll
+        for (int i = 0; i < ITEMS; ++i) rid[i] = cd.rid_base + (uint32_t)wbase + (uint32_t)(i * kWave);
+      } else {
+        const uint32_t* rp = src_rids + cd.off + wbase;
+#pragma unroll
+        for (int i = 0; i < ITEMS; ++i)
+          rid[i] = FULL || (uint32_t)(i * kWave) < nvalid ? rp[i * kWave] : 0u;
+      }
+      __syncthreads();
+      const uint32_t tile_n = FULL ? (uint32_t)TILE : s_tilen;
+      uint32_t* dk = dst_keys + cd.off;
+      uint32_t* dr = dst_rids + cd.off;
+      const uint32_t last = (uint32_t)(n_out - 1);  // stores stay in the column
+      bool bad = false;
+#pragma unroll
+      for (int j = 0; j < ITEMS; ++j) {
+        const uint32_t q = (uint32_t)j * THREADS + t;
+        if (FULL || q < tile_n) {
+          const uint32_t g = s_glob[s_digit[q]] + q;
+          bad |= g > last;
+          dk[g < last ? g : last] = s_stage[q];
+        }
+      }
+      __syncthreads();
+#pragma unroll
+      for (int i = 0; i < ITEMS; ++i)
+        if (FULL || ((part >> i) & 1u)) s_stage[(slot2[i / 2] >> (16 * (i % 2))) & 0xFFFFu] = rid[i];
+      __syncthreads();
+#pragma unroll
+      for (int j = 0; j < ITEMS; ++j) {
+        const uint32_t q = (uint32_t)j * THREADS + t;
+        if (FULL || q < tile_n) {
+          const uint32_t g = s_glob[s_digit[q]] + q;
+          dr[g < last ? g : last] = s_stage[q];
+        }
+      }
+      if (RUN) {
+        // literal bits of each run item, moved along with its key and rid
+        __syncthreads();
+        const uint32_t* lp = src_lits + cd.off + wbase;  // one base, immediate offsets
+#pragma unroll
+        for (int i = 0; i < ITEMS; ++i)
+          if (FULL || ((part >> i) & 1u)) s_stage[(slot2[i / 2] >> (16 * (i % 2))) & 0xFFFFu] = lp[i * kWave];
+        __syncthreads();
+        uint32_t* dl = dst_lits + cd.off;
+#pragma unroll 4  // a full unroll keeps the slot indices of all three rounds live
+        for (int j = 0; j < ITEMS; ++j) {
+          const uint32_t q = (uint32_t)j * THREADS + t;
+          if (FULL || q < tile_n) {
+            const uint32_t g = s_glob[s_digit[q]] + q;
+            dl[g < last ? g : last] = s_stage[q];
+          }
+        }
+      }
+      if (ballot(bad) && l == 0) atomicOr(fault, 2u);  // a scatter index past the column
+    };
+    if (base + TILE <= n) sweep(std::true_type{});
+    else sweep(std::false_type{});
     __syncthreads();
   }
 }

@@ -127,7 +127,6 @@ __global__ __launch_bounds__(kBlock) void k_histogram(
   __shared__ uint32_t h[4 * 256];
   __shared__ uint32_t hr[4 * 256];
   __shared__ uint32_t s_max[kWaves];
-  __shared__ uint32_t s_reps[kWaves];
   const int t = threadIdx.x;
   const Dims dims = desc->dims;
   const Column* cols = desc_cols(desc);
@@ -220,17 +219,25 @@ __global__ __launch_bounds__(kBlock) void k_histogram(
         hist_value(h, hr, v, valid, rep);
       }
     }
-    // runs of the chunk, added to its radix tile (a run-mode column's cut
-    // places its items from these counts)
+    // runs of the wave's rows, added to their radix tile (a run-mode
+    // column's cut places its items from these counts); no block barrier,
+    // so the waves of a block stream their chunks independently
     reps = (uint32_t)wave_sum64(reps);
-    if ((t % kWave) == 0) s_reps[t / kWave] = reps;
-    __syncthreads();
-    if (t == 0) {
-      uint32_t r = 0;
-      for (int i = 0; i < kWaves; ++i) r += s_reps[i];
-      atomicAdd(&tile_runs[cd.sort_tile0 + (uint32_t)(lo / sort_tile)], cnt - r);
+    if ((t % kWave) == 0) {
+      // rows this wave covered: the fast path gives wave w rows 4 x 64 x
+      // {u * 256 + w * 64 ..}; the slow path row i to thread i % 256
+      const uint32_t w = t / kWave;
+      uint32_t rows = 0;
+      if (cnt == kHistChunk && ((uintptr_t)src % 16) == 0) {
+        rows = kHistChunk / kWaves;
+      } else {
+        for (uint32_t b = 0; b < cnt; b += kBlock) {
+          const uint32_t lo_i = b + w * kWave;
+          rows += cnt > lo_i ? (cnt - lo_i < (uint32_t)kWave ? cnt - lo_i : (uint32_t)kWave) : 0u;
+        }
+      }
+      if (rows - reps) atomicAdd(&tile_runs[cd.sort_tile0 + (uint32_t)(lo / sort_tile)], rows - reps);
     }
-    __syncthreads();
   }
   hist_flush(h, hr, s_max, vmax, &cs[col]);
 }

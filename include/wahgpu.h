@@ -94,7 +94,7 @@ uint64_t wahgpu_arena_bytes(const wahgpu_ctx* ctx);
  * wahgpu_create shows the device returns same-address fetch-adds of one wave
  * instruction in lane order, which makes the rank stable), 0 = ballot match.
  * The environment variable WAHGPU_RANK=match forces 0. */
-int wahgpu_rank_mode(const wahgpu_ctx* ctx); /* currently reserved */
+int wahgpu_rank_mode(const wahgpu_ctx* ctx);
 
 /* ---- whole build (the indexer's worker, src/gpu.cpp:356-367; the phases
  * pipeline, src/phases.cpp:720-1088) ---------------------------------------
@@ -215,11 +215,12 @@ int wahgpu_set_tuning(wahgpu_ctx* ctx, const wahgpu_tuning* in);
  * (src/gpu.cpp:194-261): requests (one column each, host memory) queue as
  * they arrive; `slots` worker threads, each with its own stream, device
  * buffers and captured graph, take every queued request that fits their
- * batch (admission by HBM footprint: ~45 bytes per value plus ~8.5 KiB per
+ * batch (admission by HBM footprint: 49 bytes per value plus ~36 KiB per
  * column, within arena_bytes / slots) and run it as one batched build, so
  * requests arriving while others are on the GPU join the next batch of the
  * first slot to free up, and the copies and kernels of different slots
- * overlap on the device.  With plain = 1 every slot instead serves one
+ * overlap on the device.  A request completes as soon as its own result
+ * copies have ended, in batch order.  With plain = 1 every slot instead serves one
  * request at a time through wahgpu_build (the per-column drop-in), for
  * comparison under the same arrivals. */
 typedef struct wahgpu_server wahgpu_server;

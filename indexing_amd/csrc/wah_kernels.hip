@@ -810,7 +810,9 @@ __global__ __launch_bounds__(THREADS, kTailWavesPerEU) void k_tail(
   static_assert(ITEMS % 4 == 0 && ITEMS >= 8 && ITEMS <= 16, "16-byte loads; runs span <= 4 lanes");
   static_assert(2 * TILE < 65536, "tile word and key counts fit 16 bits");
   constexpr uint32_t kAll = (1u << ITEMS) - 1u;
-  __shared__ uint32_t s_words[2 * TILE];
+  // staged words, one pad word per 16: lanes that write ~16 words apart
+  // (two per item) land in different banks
+  __shared__ uint32_t s_words[2 * TILE + 2 * TILE / 16];
   __shared__ uint32_t s_wtot[WAVES];
   __shared__ uint32_t s_wlead[WAVES + 1];  // [w]: lead bits of wave w; [WAVES]: of the next tile
   __shared__ uint64_t s_prefix;
@@ -1002,9 +1004,9 @@ __global__ __launch_bounds__(THREADS, kTailWavesPerEU) void k_tail(
         const bool st = (stm >> i) & 1u, fw = (fm >> i) & 1u;
         carry |= b[i];
         pos -= st ? 1u : 0u;
-        if (st) s_words[pos] = 0x80000000u | carry;
+        if (st) s_words[pos + (pos >> 4)] = 0x80000000u | carry;
         pos -= fw ? 1u : 0u;
-        if (fw) s_words[pos] = f[i];
+        if (fw) s_words[pos + (pos >> 4)] = f[i];
         carry = st ? 0u : carry;
       }
     }
@@ -1036,7 +1038,7 @@ __global__ __launch_bounds__(THREADS, kTailWavesPerEU) void k_tail(
     const uint32_t Wt = tot & 0xFFFFu;
     for (uint32_t j = t; j < Wt; j += THREADS) {
       const uint32_t o = Wpre + j;
-      if (o < 2 * n) out_words[o] = s_words[j];
+      if (o < 2 * n) out_words[o] = s_words[j + (j >> 4)];
       else atomicOr(fault, 2u);
     }
     if (base + TILE >= n && t == 0) {  // the column's last tile

@@ -52,6 +52,16 @@ constexpr int kTailTile = kTailThreads * kTailItems;  // sorted pairs per tail t
 #endif
 constexpr int kTailWavesPerEU = WAH_TAIL_WAVES_PER_EU;
 constexpr int kTailBlocksPerCU = kTailWavesPerEU * 4 * 64 / kTailThreads;
+// Run-mode tail: the same 8192-item tiles as 512 threads x 16 items at 4
+// waves per SIMD (128 VGPRs), two blocks per CU.
+constexpr int kRunTailThreads = 512;
+constexpr int kRunTailItems = kTailTile / kRunTailThreads;
+constexpr int kRunTailBlocksPerCU = 4 * 4 * 64 / kRunTailThreads;
+static_assert(kRunTailThreads * kRunTailItems == kTailTile, "run tail tiles = pair tail tiles");
+// waves per SIMD a tail instantiation is compiled for
+__host__ __device__ constexpr int tail_waves_per_eu(int items) {
+  return items >= 16 ? 4 : kTailWavesPerEU;
+}
 constexpr int kScanItems = 8;
 constexpr int kScanTile = 256 * kScanItems;
 constexpr uint32_t kHistMaxBlocks = 2048;

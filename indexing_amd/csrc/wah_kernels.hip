@@ -546,7 +546,6 @@ __global__ __launch_bounds__(THREADS, WAH_SORT_WAVES_PER_EU) void k_onesweep(
   static_assert(ITEMS <= 32, "participation bits");
   static_assert(TILE <= 65536, "LDS slots packed in 16 bits");
   __shared__ uint32_t s_stage[TILE];
-  __shared__ uint8_t s_digit[TILE];
   __shared__ uint32_t s_wcnt[WAVES][256];
   __shared__ uint32_t s_excl[256];
   __shared__ uint32_t s_glob[256];
@@ -668,18 +667,19 @@ __global__ __launch_bounds__(THREADS, WAH_SORT_WAVES_PER_EU) void k_onesweep(
 #pragma unroll
         for (int i = 0; i < 4; ++i) before += i < w ? s_dsum[i] : 0u;
         const uint32_t ex = s_excl[t] + before;
-        s_excl[t] = ex;
         s_glob[t] -= ex;  // column index = s_glob[d] + local index
         if (t == 255) s_tilen = ex + total;
+        // each wave's first slot for digit t: one LDS read per key below
+#pragma unroll
+        for (int i = 0; i < WAVES; ++i) s_wcnt[i][t] += ex;
       }
       __syncthreads();
 #pragma unroll
       for (int i = 0; i < ITEMS; ++i) {
         if (FULL || ((part >> i) & 1u)) {
           const uint32_t d = (key[i] >> shift) & 255u;
-          const uint32_t pos = s_excl[d] + s_wcnt[w][d] + rank[i];
+          const uint32_t pos = s_wcnt[w][d] + rank[i];
           s_stage[pos] = key[i];
-          s_digit[pos] = (uint8_t)d;
           rank[i] = pos;  // keep the LDS slot for the rids
         }
       }
@@ -705,13 +705,19 @@ __global__ __launch_bounds__(THREADS, WAH_SORT_WAVES_PER_EU) void k_onesweep(
       uint32_t* dr = dst_rids + cd.off;
       const uint32_t last = (uint32_t)(n_out - 1);  // stores stay in the column
       bool bad = false;
+      // destination of each slot this thread stores, kept for the rids (and
+      // literal bits): the digit comes from the staged key
+      uint32_t gq[ITEMS];
 #pragma unroll
       for (int j = 0; j < ITEMS; ++j) {
         const uint32_t q = (uint32_t)j * THREADS + t;
+        gq[j] = last;
         if (FULL || q < tile_n) {
-          const uint32_t g = s_glob[s_digit[q]] + q;
+          const uint32_t k = s_stage[q];
+          const uint32_t g = s_glob[(k >> shift) & 255u] + q;
           bad |= g > last;
-          dk[g < last ? g : last] = s_stage[q];
+          gq[j] = g < last ? g : last;
+          dk[gq[j]] = k;
         }
       }
       __syncthreads();
@@ -722,10 +728,7 @@ __global__ __launch_bounds__(THREADS, WAH_SORT_WAVES_PER_EU) void k_onesweep(
 #pragma unroll
       for (int j = 0; j < ITEMS; ++j) {
         const uint32_t q = (uint32_t)j * THREADS + t;
-        if (FULL || q < tile_n) {
-          const uint32_t g = s_glob[s_digit[q]] + q;
-          dr[g < last ? g : last] = s_stage[q];
-        }
+        if (FULL || q < tile_n) dr[gq[j]] = s_stage[q];
       }
       if (RUN) {
         // literal bits of each run item, moved along with its key and rid
@@ -736,13 +739,10 @@ __global__ __launch_bounds__(THREADS, WAH_SORT_WAVES_PER_EU) void k_onesweep(
           if (FULL || ((part >> i) & 1u)) s_stage[(slot2[i / 2] >> (16 * (i % 2))) & 0xFFFFu] = lp[i * kWave];
         __syncthreads();
         uint32_t* dl = dst_lits + cd.off;
-#pragma unroll 4  // a full unroll keeps the slot indices of all three rounds live
+#pragma unroll
         for (int j = 0; j < ITEMS; ++j) {
           const uint32_t q = (uint32_t)j * THREADS + t;
-          if (FULL || q < tile_n) {
-            const uint32_t g = s_glob[s_digit[q]] + q;
-            dl[g < last ? g : last] = s_stage[q];
-          }
+          if (FULL || q < tile_n) dl[gq[j]] = s_stage[q];
         }
       }
       if (ballot(bad) && l == 0) atomicOr(fault, 2u);  // a scatter index past the column
@@ -798,7 +798,7 @@ __device__ __forceinline__ SortedView sorted_view(const ColState* col, const Col
 // stored contiguously.
 // ---------------------------------------------------------------------------
 template <int THREADS, int ITEMS, bool RUN>
-__global__ __launch_bounds__(THREADS, kTailWavesPerEU) void k_tail(
+__global__ __launch_bounds__(THREADS, tail_waves_per_eu(ITEMS)) void k_tail(
     const DescHead* __restrict__ desc, const ColState* __restrict__ cs, Header* __restrict__ hdr,
     const uint32_t* values, const uint32_t* keysA, const uint32_t* ridsA,
     const uint32_t* keysB, const uint32_t* ridsB, const uint32_t* litsA, const uint32_t* litsB,
@@ -1368,12 +1368,12 @@ static uint32_t cap_grid(uint32_t work, uint32_t cap) {
 }
 
 // Radix blocks resident per CU: the waves the register budget allows
-// (WAH_SORT_WAVES_PER_EU per SIMD) and the LDS of k_onesweep (staged keys
-// and digits, per-wave digit counts, digit tables; 160 KiB per CU).
+// (WAH_SORT_WAVES_PER_EU per SIMD) and the LDS of k_onesweep (staged keys,
+// per-wave digit counts, digit tables; 160 KiB per CU).
 template <int T, int I>
 constexpr uint32_t sort_blocks_per_cu() {
   constexpr uint32_t by_waves = WAH_SORT_WAVES_PER_EU * 4 / (T / kWave);
-  constexpr uint32_t lds = T * I * 5 + (T / kWave) * 1024 + 2 * 1024 + 64;
+  constexpr uint32_t lds = T * I * 4 + (T / kWave) * 1024 + 2 * 1024 + 64;
   constexpr uint32_t by_lds = 160u * 1024u / lds;
   constexpr uint32_t b = by_waves < by_lds ? by_waves : by_lds;
   return b ? b : 1u;
@@ -1437,10 +1437,15 @@ void launch_tail(const BuildArgs& a, hipStream_t s, LaunchHook* hook) {
                      0, s, a.desc, a.cs, a.hdr, a.values, a.keysA, a.ridsA, a.keysB, a.ridsB,
                      a.litsA, a.litsB, a.out_keys, a.out_offsets, a.out_words, a.tail_status,
                      lanes, a.counts);
-  hipLaunchKernelGGL((k_tail<kTailThreads, kTailItems, true>), dim3(grid), dim3(kTailThreads),
-                     0, s, a.desc, a.cs, a.hdr, a.values, a.keysA, a.ridsA, a.keysB, a.ridsB,
-                     a.litsA, a.litsB, a.out_keys, a.out_offsets, a.out_words, a.tail_status,
-                     lanes, a.counts);
+  // run mode: the same tiles as 512 threads x 16 items (fewer, longer lanes
+  // suit its sparser items)
+  const uint32_t rgrid =
+      cap_grid(a.tail_tiles_cap, WAH_STATIC_TILES ? a.tail_tiles_cap
+                                                  : a.max_resident / 4 * kRunTailBlocksPerCU);
+  hipLaunchKernelGGL((k_tail<kRunTailThreads, kRunTailItems, true>), dim3(rgrid),
+                     dim3(kRunTailThreads), 0, s, a.desc, a.cs, a.hdr, a.values, a.keysA, a.ridsA,
+                     a.keysB, a.ridsB, a.litsA, a.litsB, a.out_keys, a.out_offsets, a.out_words,
+                     a.tail_status, WAH_TAIL_LANES ? WAH_TAIL_LANES : tile_lanes(rgrid), a.counts);
   if (hook) hook->mark(s);
   uint64_t lg = (a.n_cap + kBlock - 1) / kBlock;
   hipLaunchKernelGGL(k_finish, dim3(cap_grid(lg > 1024 ? 1024u : (uint32_t)lg, 1024)),

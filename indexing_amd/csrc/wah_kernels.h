@@ -156,6 +156,7 @@ struct BuildArgs {
   uint32_t* sort_status;             // 2 arrays of sort_status_stride words
   uint64_t sort_status_stride;
   uint64_t* tail_status;             // tail_tiles_cap words
+  uint32_t* tile_entries;            // tail_tiles_cap words: entries (k) per tail tile
   uint32_t* tile_runs;               // sort_tiles_cap words: runs per radix tile, then
                                      // their exclusive prefix within the column
   uint32_t *out_keys, *out_offsets, *out_lengths, *out_words;

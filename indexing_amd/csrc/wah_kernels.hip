@@ -123,7 +123,8 @@ __device__ __forceinline__ void hist_flush(uint32_t* h, uint32_t* hr, uint32_t* 
 __global__ __launch_bounds__(kBlock) void k_histogram(
     const uint32_t* __restrict__ values, const DescHead* __restrict__ desc,
     ColState* __restrict__ cs, uint32_t* __restrict__ status0,
-    uint64_t* __restrict__ tail_status, uint32_t* __restrict__ tile_runs, uint32_t sort_tile) {
+    uint64_t* __restrict__ tail_status, uint32_t* __restrict__ tile_entries,
+    uint32_t* __restrict__ tile_runs, uint32_t sort_tile) {
   __shared__ uint32_t h[4 * 256];
   __shared__ uint32_t hr[4 * 256];
   __shared__ uint32_t s_max[kWaves];
@@ -136,7 +137,10 @@ __global__ __launch_bounds__(kBlock) void k_histogram(
   const uint64_t gt = (uint64_t)blockIdx.x * kBlock + t;
   const uint64_t s0 = (uint64_t)dims.sort_tiles * 256;
   for (uint64_t i = gt; i < s0; i += gthreads) status0[i] = 0;
-  for (uint64_t i = gt; i < dims.tail_tiles; i += gthreads) tail_status[i] = 0;
+  for (uint64_t i = gt; i < dims.tail_tiles; i += gthreads) {
+    tail_status[i] = 0;
+    tile_entries[i] = 0;
+  }
   __syncthreads();
 
   const uint32_t per = (dims.hist_chunks + gridDim.x - 1) / gridDim.x;
@@ -803,8 +807,8 @@ __global__ __launch_bounds__(THREADS, tail_waves_per_eu(ITEMS)) void k_tail(
     const uint32_t* values, const uint32_t* keysA, const uint32_t* ridsA,
     const uint32_t* keysB, const uint32_t* ridsB, const uint32_t* litsA, const uint32_t* litsB,
     uint32_t* __restrict__ keys_out, uint32_t* __restrict__ offsets_out,
-    uint32_t* __restrict__ words_out, uint64_t* status, uint32_t lanes,
-    wahgpu_counts_dev* __restrict__ counts) {
+    uint32_t* __restrict__ words_out, uint64_t* status, uint32_t* __restrict__ tile_entries,
+    uint32_t lanes, wahgpu_counts_dev* __restrict__ counts) {
   constexpr int TILE = THREADS * ITEMS;
   constexpr int WAVES = THREADS / kWave;
   static_assert(ITEMS % 4 == 0 && ITEMS >= 8 && ITEMS <= 16, "16-byte loads; runs span <= 4 lanes");
@@ -814,6 +818,7 @@ __global__ __launch_bounds__(THREADS, tail_waves_per_eu(ITEMS)) void k_tail(
   // (two per item) land in different banks
   __shared__ uint32_t s_words[2 * TILE + 2 * TILE / 16];
   __shared__ uint32_t s_wtot[WAVES];
+  __shared__ uint32_t s_went[WAVES];
   __shared__ uint32_t s_wlead[WAVES + 1];  // [w]: lead bits of wave w; [WAVES]: of the next tile
   __shared__ uint64_t s_prefix;
   __shared__ uint32_t s_tile;
@@ -821,16 +826,7 @@ __global__ __launch_bounds__(THREADS, tail_waves_per_eu(ITEMS)) void k_tail(
   const Column* cols = desc_cols(desc);
   const int t = threadIdx.x, w = t / kWave, l = t % kWave;
   uint32_t* fault = &hdr->fault;
-  uint32_t entries = 0;           // entries of column ent_col counted by this thread
-  uint32_t ent_col = 0xFFFFFFFFu;
   if (hdr->mode_cols[RUN ? 1 : 0] == 0) return;  // no column in this mode
-  // k of a column: one atomic per wave and column
-  auto flush_entries = [&]() {
-    const uint64_t e = wave_sum64((uint64_t)entries);
-    if (l == 0 && e)
-      atomicAdd((unsigned long long*)&counts[cols[ent_col].index].k, (unsigned long long)e);
-    entries = 0;
-  };
 
   uint32_t iter = 0;
   for (;;) {
@@ -848,10 +844,6 @@ __global__ __launch_bounds__(THREADS, tail_waves_per_eu(ITEMS)) void k_tail(
       if (RUN && c + 1 == dims.ncols && cs[c].mode == kModeRun) break;  // past the last items
       __syncthreads();  // s_tile is read by every wave before it is rewritten
       continue;
-    }
-    if (c != ent_col) {  // block-uniform
-      if (ent_col != 0xFFFFFFFFu) flush_entries();
-      ent_col = c;
     }
     const uint32_t rid_base = cd.rid_base;
     const SortedView sv = sorted_view(&cs[c], cd, values, keysA, ridsA, keysB, ridsB, litsA,
@@ -925,7 +917,6 @@ __global__ __launch_bounds__(THREADS, tail_waves_per_eu(ITEMS)) void k_tail(
       fm |= f[i] != 0u ? 1u << i : 0u;
     }
     const uint32_t nwords = (uint32_t)(__builtin_popcount(stm) + __builtin_popcount(fm));
-    entries += (uint32_t)__builtin_popcount(stm);
     // lead bits: items before the lane's first stop (entry start or end of items)
     const uint32_t stops = stm | (~0u << nv & kAll);
     const uint32_t fs = stops ? (uint32_t)__builtin_ctz(stops) : (uint32_t)ITEMS;
@@ -975,6 +966,11 @@ __global__ __launch_bounds__(THREADS, tail_waves_per_eu(ITEMS)) void k_tail(
     const uint32_t packed = ((uint32_t)__builtin_popcount(khm) << 16) | nwords;
     const uint32_t inc = wave_incl_add(packed);
     if (l == kWave - 1) s_wtot[w] = inc;
+    {
+      // entries of the tile (k of the column is their sum, formed by k_finish)
+      const uint32_t e = readlane(wave_incl_add((uint32_t)__builtin_popcount(stm)), kWave - 1);
+      if (l == 0) s_went[w] = e;
+    }
     __syncthreads();
 
     // ---- literals: reverse sweep along the lane, seeded with the bits the
@@ -993,7 +989,11 @@ __global__ __launch_bounds__(THREADS, tail_waves_per_eu(ITEMS)) void k_tail(
     if (w == 0) {
       const uint64_t tile_agg = ((uint64_t)(tot >> 16) << 31) | (tot & 0xFFFFu);
       const uint64_t pre = lookback_wave_sum(status, tile, first_tile, tile_agg, fault);
-      if (l == 0) s_prefix = pre;
+      const uint32_t e = readlane(wave_incl_add(l < WAVES ? s_went[l] : 0u), kWave - 1);
+      if (l == 0) {
+        s_prefix = pre;
+        tile_entries[tile] = e;
+      }
     }
     {
       // from the lane's last word back: [fill,] literal of each entry start
@@ -1017,8 +1017,20 @@ __global__ __launch_bounds__(THREADS, tail_waves_per_eu(ITEMS)) void k_tail(
     uint32_t* out_offsets = offsets_out + cd.off;
     uint32_t* out_words = words_out + 2 * cd.off;
     if (khm) {
-      // key starts: the value (read again, the lines are still in L2) and
-      // the position of the key's first word
+      // key starts: the values (the lane's keys read again, all at once:
+      // the lines are still in L2) and the position of each key's first word
+      uint32_t kv[ITEMS];
+      if (nv == (uint32_t)ITEMS && aligned) {
+#pragma unroll
+        for (int q = 0; q < ITEMS / 4; ++q) {
+          const uint4 k4 = reinterpret_cast<const uint4*>(sv.keys + j0)[q];
+          kv[4 * q] = k4.x; kv[4 * q + 1] = k4.y; kv[4 * q + 2] = k4.z; kv[4 * q + 3] = k4.w;
+        }
+      } else {
+#pragma unroll
+        for (int i = 0; i < ITEMS; ++i) kv[i] = (uint32_t)i < nv ? sv.keys[j0 + i] : 0u;
+      }
+      bool bad = false;
 #pragma unroll
       for (int i = 0; i < ITEMS; ++i) {
         if ((khm >> i) & 1u) {
@@ -1026,14 +1038,13 @@ __global__ __launch_bounds__(THREADS, tail_waves_per_eu(ITEMS)) void k_tail(
           const uint32_t kg = Kpre + (lpos >> 16) + __builtin_popcount(khm & below);
           const uint32_t wp = (lpos & 0xFFFFu) + __builtin_popcount(stm & below) +
                               __builtin_popcount(fm & below);
-          if (kg < n) {
-            out_keys[kg] = sv.keys[j0 + i];
-            out_offsets[kg] = Wpre + wp;
-          } else {
-            atomicOr(fault, 2u);
-          }
+          bad |= kg >= n;
+          const uint32_t kc = kg < n ? kg : n - 1u;  // stores stay in the column
+          out_keys[kc] = kv[i];
+          out_offsets[kc] = Wpre + wp;
         }
       }
+      if (bad) atomicOr(fault, 2u);
     }
     const uint32_t Wt = tot & 0xFFFFu;
     for (uint32_t j = t; j < Wt; j += THREADS) {
@@ -1048,13 +1059,14 @@ __global__ __launch_bounds__(THREADS, tail_waves_per_eu(ITEMS)) void k_tail(
     // the next tile's first barrier (in next_tile) orders these LDS reads
     // before its writes
   }
-  if (ent_col != 0xFFFFFFFFu) flush_entries();
 }
 
-// Per column: lengths[i] = offsets[i+1] - offsets[i], the last key running to
-// L; the build's fault word is copied into every column's counts.
+// Per column: k = the sum of its tail tiles' entry counts; lengths[i] =
+// offsets[i+1] - offsets[i], the last key running to L; the build's fault
+// word is copied into every column's counts.
 __global__ __launch_bounds__(kBlock) void k_finish(const DescHead* __restrict__ desc,
                                                    const Header* __restrict__ hdr,
+                                                   const uint32_t* __restrict__ tile_entries,
                                                    const uint32_t* __restrict__ offsets,
                                                    uint32_t* __restrict__ lengths,
                                                    wahgpu_counts_dev* __restrict__ counts) {
@@ -1063,6 +1075,31 @@ __global__ __launch_bounds__(kBlock) void k_finish(const DescHead* __restrict__ 
   const uint32_t fault = hdr->fault;
   const uint64_t gt = (uint64_t)blockIdx.x * kBlock + threadIdx.x;
   const uint64_t gs = (uint64_t)gridDim.x * kBlock;
+  {
+    // k per column = the entry counts of its tail tiles (zeroed by the
+    // histogram, written by the tiles that held items).  Each wave takes 64
+    // consecutive tiles, sums them per column (segmented wave scan) and adds
+    // each column's part once.
+    const uint32_t lane = threadIdx.x % kWave;
+    const uint32_t ntiles = desc->dims.tail_tiles;
+    for (uint64_t w0 = gt - lane; w0 < ntiles; w0 += gs) {
+      const uint32_t tile = (uint32_t)w0 + lane;
+      uint32_t c = find_segment(&cols[0].tail_tile0, sizeof(Column) / 4, ncols, (uint32_t)w0);
+      uint32_t v = 0;
+      if (tile < ntiles) {
+        while (c + 1 < ncols && cols[c + 1].tail_tile0 <= tile) ++c;
+        v = tile_entries[tile];
+      }
+      const uint32_t cp = (uint32_t)__builtin_amdgcn_update_dpp(-1, (int)c, 0x138, 0xF, 0xF, false);
+      const uint64_t hm = ballot(c != cp);  // first lane of each column
+      const uint32_t inc = wave_incl_add(v);
+      const uint64_t upto = lane == kWave - 1 ? ~0ull : ((2ull << lane) - 1ull);
+      const int h = 63 - __builtin_clzll(hm & upto);  // this lane's first lane of its column
+      const uint32_t seg = inc - (uint32_t)__shfl((int)(inc - v), h, kWave);
+      const bool last = lane == kWave - 1 || ((hm >> (lane + 1)) & 1ull);
+      if (last && seg) atomicAdd((unsigned long long*)&counts[cols[c].index].k, (unsigned long long)seg);
+    }
+  }
   for (uint32_t c = 0; c < ncols; ++c) {
     const Column cd = cols[c];
     const uint64_t kc = counts[cd.index].keycnt;
@@ -1417,7 +1454,8 @@ void launch_sort(const BuildArgs& a, hipStream_t s, LaunchHook* hook) {
   uint64_t hb = (a.n_cap + 4 * kHistChunk - 1) / (4 * kHistChunk);
   if (hb > a.hist_chunks_cap) hb = a.hist_chunks_cap;
   hipLaunchKernelGGL(k_histogram, dim3(cap_grid((uint32_t)hb, kHistMaxBlocks)), dim3(kBlock), 0,
-                     s, a.values, a.desc, a.cs, a.sort_status, a.tail_status, a.tile_runs,
+                     s, a.values, a.desc, a.cs, a.sort_status, a.tail_status, a.tile_entries,
+                     a.tile_runs,
                      radix_tile(a.shape));
   hipLaunchKernelGGL(k_plan, dim3(cap_grid(a.cols_cap, 1024)), dim3(kBlock), 0, s, a.desc, a.cs,
                      a.hdr, a.tile_runs, a.counts, (int)a.force_pair, radix_tile(a.shape));
@@ -1436,7 +1474,7 @@ void launch_tail(const BuildArgs& a, hipStream_t s, LaunchHook* hook) {
   hipLaunchKernelGGL((k_tail<kTailThreads, kTailItems, false>), dim3(grid), dim3(kTailThreads),
                      0, s, a.desc, a.cs, a.hdr, a.values, a.keysA, a.ridsA, a.keysB, a.ridsB,
                      a.litsA, a.litsB, a.out_keys, a.out_offsets, a.out_words, a.tail_status,
-                     lanes, a.counts);
+                     a.tile_entries, lanes, a.counts);
   // run mode: the same tiles as 512 threads x 16 items (fewer, longer lanes
   // suit its sparser items)
   const uint32_t rgrid =
@@ -1445,11 +1483,13 @@ void launch_tail(const BuildArgs& a, hipStream_t s, LaunchHook* hook) {
   hipLaunchKernelGGL((k_tail<kRunTailThreads, kRunTailItems, true>), dim3(rgrid),
                      dim3(kRunTailThreads), 0, s, a.desc, a.cs, a.hdr, a.values, a.keysA, a.ridsA,
                      a.keysB, a.ridsB, a.litsA, a.litsB, a.out_keys, a.out_offsets, a.out_words,
-                     a.tail_status, WAH_TAIL_LANES ? WAH_TAIL_LANES : tile_lanes(rgrid), a.counts);
+                     a.tail_status, a.tile_entries, WAH_TAIL_LANES ? WAH_TAIL_LANES : tile_lanes(rgrid),
+                     a.counts);
   if (hook) hook->mark(s);
   uint64_t lg = (a.n_cap + kBlock - 1) / kBlock;
   hipLaunchKernelGGL(k_finish, dim3(cap_grid(lg > 1024 ? 1024u : (uint32_t)lg, 1024)),
-                     dim3(kBlock), 0, s, a.desc, a.hdr, a.out_offsets, a.out_lengths, a.counts);
+                     dim3(kBlock), 0, s, a.desc, a.hdr, a.tile_entries, a.out_offsets,
+                     a.out_lengths, a.counts);
 }
 
 void launch_sort_finalize(const BuildArgs& a, hipStream_t s) {

@@ -146,6 +146,7 @@ uint64_t build_scratch_bytes(const Caps& k, uint32_t ndesc = 1) {
   b += align_up((uint64_t)k.cols * sizeof(wah::ColState));
   b += align_up(2ull * k.sort_tiles * 256 * 4);
   b += align_up((uint64_t)k.tail_tiles * 8 + 8);
+  b += align_up((uint64_t)k.tail_tiles * 4 + 4);
   b += align_up((uint64_t)k.sort_tiles * 4 + 4);
   return b;
 }
@@ -167,6 +168,7 @@ void carve_build(wahgpu_ctx* c, BuildArgs* a, const Caps& k, int shape) {
   a->sort_status_stride = (uint64_t)k.sort_tiles * 256;
   a->sort_status = take<uint32_t>(c, 2 * a->sort_status_stride);
   a->tail_status = take<uint64_t>(c, (uint64_t)k.tail_tiles + 1);
+  a->tile_entries = take<uint32_t>(c, (uint64_t)k.tail_tiles + 1);
   a->tile_runs = take<uint32_t>(c, (uint64_t)k.sort_tiles + 1);
   a->max_resident = c->max_resident;
   a->atomic_rank = c->atomic_rank;

@@ -190,14 +190,15 @@ int wahgpu_graph_stats(const wahgpu_ctx* ctx, uint64_t* captures, uint64_t* repl
 /* Tile shapes per column-length bucket.  A build sorts each column with the
  * radix tile shape of its length bucket (columns of different shapes in one
  * batch run as one launch set per shape):
- *   buckets:  n < 2^14, n < 2^17, n < 2^20, larger;
+ *   buckets:  n < 2^14, n < 2^17, n < 2^20, n < 2^24, larger;
  *   shapes:   0 = 1024 threads x 16 keys, 1 = 512 x 16, 2 = 256 x 16
  *             (8-bit digits in every shape; the tail tile is 1024 x 8).
  * wahgpu_autotune times every shape on synthetic batches of each bucket
- * (uniform values, 2^12 distinct, about 2^24 values per bucket) on this
- * device, keeps the fastest per bucket in the context and reports it with
- * the timings.  The default table is shape 0 everywhere. */
-#define WAHGPU_TUNE_BUCKETS 4
+ * (uniform values, 2^12 distinct; columns of 2^13, 2^15.5, 2^18.5, 2^22 and
+ * 2^26 values, 2^24 values per bucket and one 2^26 column for the last) on
+ * this device, keeps the fastest per bucket in the context and reports it
+ * with the timings.  The default table is shape 0 everywhere. */
+#define WAHGPU_TUNE_BUCKETS 5
 #define WAHGPU_TUNE_SHAPES 3
 typedef struct wahgpu_tuning {
   uint32_t bucket_log2[WAHGPU_TUNE_BUCKETS]; /* bucket b: n < 2^bucket_log2[b] (64: any) */

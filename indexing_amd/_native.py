@@ -68,7 +68,7 @@ class ServerStats(ctypes.Structure):
                 ("max_us", ctypes.c_double), ("busy_s", ctypes.c_double)]
 
 
-TUNE_BUCKETS, TUNE_SHAPES = 4, 3
+TUNE_BUCKETS, TUNE_SHAPES = 5, 3
 
 
 class Tuning(ctypes.Structure):

@@ -46,7 +46,7 @@ struct wahgpu_ctx {
   uint64_t captures = 0, replays = 0;
   bool atomic_rank = false;  // radix ranking by LDS fetch-add (self-tested)
   // radix tile shape per column-length bucket (wahgpu_autotune / set_tuning)
-  uint32_t tune_shape[WAHGPU_TUNE_BUCKETS] = {0, 0, 0, 0};
+  uint32_t tune_shape[WAHGPU_TUNE_BUCKETS] = {0, 0, 0, 0, 0};
   double tune_ms[WAHGPU_TUNE_BUCKETS][WAHGPU_TUNE_SHAPES] = {};
   int force_shape = -1;  // autotune runs: every column in this shape
 };
@@ -120,7 +120,7 @@ struct Caps {
   }
 };
 // Column-length bucket of n: < 2^14, < 2^17, < 2^20, the rest.
-constexpr uint32_t kBucketLog2[WAHGPU_TUNE_BUCKETS] = {14, 17, 20, 64};
+constexpr uint32_t kBucketLog2[WAHGPU_TUNE_BUCKETS] = {14, 17, 20, 24, 64};
 uint32_t bucket_of(uint64_t n) {
   for (uint32_t b = 0; b + 1 < WAHGPU_TUNE_BUCKETS; ++b)
     if (n < (1ull << kBucketLog2[b])) return b;
@@ -885,14 +885,15 @@ int wahgpu_set_tuning(wahgpu_ctx* c, const wahgpu_tuning* in) {
 }
 
 // Times every tile shape on a synthetic batch of each length bucket and keeps
-// the fastest: columns of one length per bucket (2^13, 2^15.5, 2^18.5, 2^22
-// values), uniform over 4096 values, about 2^24 values per bucket.
+// the fastest: columns of one length per bucket (2^13, 2^15.5, 2^18.5, 2^22,
+// 2^26 values), uniform over 4096 values, 2^24 values per bucket (one 2^26
+// column for the last).
 int wahgpu_autotune(wahgpu_ctx* c, wahgpu_tuning* out) {
   if (!c) return WAHGPU_EINVAL;
   HIPCHK(c, hipSetDevice(c->device));
   drop_graph(c);
-  const double len_log2[WAHGPU_TUNE_BUCKETS] = {13.0, 15.5, 18.5, 22.0};
-  const uint64_t total = 1ull << 24;
+  const double len_log2[WAHGPU_TUNE_BUCKETS] = {13.0, 15.5, 18.5, 22.0, 26.0};
+  const uint64_t total = 1ull << 26;  // device buffers: the largest bucket's values
   uint32_t *d_v = nullptr, *d_k = nullptr, *d_o = nullptr, *d_l = nullptr, *d_w = nullptr;
   wahgpu_counts* d_c = nullptr;
   const uint64_t maxcols = 4096;
@@ -922,7 +923,8 @@ int wahgpu_autotune(wahgpu_ctx* c, wahgpu_tuning* out) {
   for (uint32_t b = 0; b < WAHGPU_TUNE_BUCKETS && rc == WAHGPU_OK; ++b) {
     const uint64_t len = (uint64_t)std::pow(2.0, len_log2[b]);
     const uint64_t step = (len + 63) / 64 * 64;
-    const uint32_t ncols = (uint32_t)std::min<uint64_t>(maxcols, total / step);
+    const uint64_t values = std::max<uint64_t>(len, 1ull << 24);  // per bucket
+    const uint32_t ncols = (uint32_t)std::min<uint64_t>(maxcols, values / step);
     std::vector<wahgpu_column> cols(ncols);
     for (uint32_t i = 0; i < ncols; ++i) cols[i] = wahgpu_column{i * step, len, 0u, 0u};
     wahgpu_gen_params gp{};
@@ -930,9 +932,9 @@ int wahgpu_autotune(wahgpu_ctx* c, wahgpu_tuning* out) {
     gp.from = 0;
     gp.to = 4095;
     gp.seed = 1000 + b;
-    if (wahgpu_generate(&gp, host.data(), total) != WAHGPU_OK) rc = WAHGPU_EINVAL;
+    if (wahgpu_generate(&gp, host.data(), values) != WAHGPU_OK) rc = WAHGPU_EINVAL;
     if (rc == WAHGPU_OK &&
-        hipMemcpy(d_v, host.data(), total * 4, hipMemcpyHostToDevice) != hipSuccess)
+        hipMemcpy(d_v, host.data(), values * 4, hipMemcpyHostToDevice) != hipSuccess)
       rc = WAHGPU_EHIP;
     double best = 1e30;
     for (int sh = 0; sh < WAHGPU_TUNE_SHAPES && rc == WAHGPU_OK; ++sh) {
@@ -940,7 +942,7 @@ int wahgpu_autotune(wahgpu_ctx* c, wahgpu_tuning* out) {
       float ms_best = 1e30f;
       for (int rep = 0; rep < 4 && rc == WAHGPU_OK; ++rep) {
         (void)hipEventRecord(ev[0], c->stream);
-        rc = batch_device_impl(c, d_v, total, cols.data(), ncols, d_k, d_o, d_l, d_w, d_c, 0);
+        rc = batch_device_impl(c, d_v, values, cols.data(), ncols, d_k, d_o, d_l, d_w, d_c, 0);
         (void)hipEventRecord(ev[1], c->stream);
         if (rc == WAHGPU_OK && hipEventSynchronize(ev[1]) != hipSuccess) rc = WAHGPU_EHIP;
         float ms = 0.f;

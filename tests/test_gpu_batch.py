@@ -12,6 +12,7 @@ import numpy as np
 import pytest
 
 import indexing_amd as ia
+import indexing_amd._native as N
 from oracle import oracle
 from tests.test_gpu_parity import assert_same, dev, u32
 
@@ -153,8 +154,8 @@ def test_every_tile_shape_matches_oracle(shape):
     """Each compiled radix tile shape, for single columns and a batch."""
     c = ia.Context(0)
     try:
-        c.set_tuning([shape] * 4)
-        assert c.tuning()["shape"] == [shape] * 4
+        c.set_tuning([shape] * N.TUNE_BUCKETS)
+        assert c.tuning()["shape"] == [shape] * N.TUNE_BUCKETS
         rng = np.random.default_rng(40 + shape)
         singles = [rng.integers(0, 65536, 100003).astype(np.uint32),
                    rng.integers(0, 1 << 32, 9000, dtype=np.uint64).astype(np.uint32),
@@ -180,7 +181,7 @@ def test_mixed_shapes_in_one_batch_and_graph():
     import torch
     c = ia.Context(0)
     try:
-        c.set_tuning([2, 1, 0, 1])
+        c.set_tuning([2, 1, 0, 1, 2])
         cols = mixed_columns(71, 40, max_log=22)
         buf, layout = pack(cols)
         ref = [oracle.build(col) for col in cols]
@@ -206,8 +207,8 @@ def test_mixed_shapes_in_one_batch_and_graph():
 def test_autotune_reports_a_table(ctx):
     t = ctx.autotune()
     print(f"[autotune] {t}", flush=True)
-    assert t["bucket_log2"] == [14, 17, 20, 64] and t["digit_bits"] == [8] * 4
-    for b in range(4):
+    assert t["bucket_log2"] == [14, 17, 20, 24, 64] and t["digit_bits"] == [8] * 5
+    for b in range(5):
         assert t["shape"][b] in (0, 1, 2)
         assert all(ms > 0 for ms in t["ms"][b])
         assert t["ms"][b][t["shape"][b]] == min(t["ms"][b])

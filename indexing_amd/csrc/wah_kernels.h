@@ -141,7 +141,8 @@ struct ColState {
   uint32_t shift[4];              // digit shift of its a-th active pass
   uint32_t mode;                  // kModePair / kModeRun
   uint32_t items;                 // items sorted: n, or the runs in run mode
-  uint32_t pad[55];
+  uint32_t narrow;                // 1: keys < 2^16 move as 16-bit words between passes
+  uint32_t pad[54];
   uint32_t digit_offset[4][256];  // exclusive prefix of each item digit histogram
 };
 constexpr uint32_t kColZeroWords = 8 * 256 + 1;

@@ -318,6 +318,9 @@ __global__ __launch_bounds__(kBlock) void k_plan(const DescHead* __restrict__ de
       for (uint32_t q = a; q < 4; ++q) col->shift[q] = 0;
       col->num_active = a;
       col->mode = run_mode ? kModeRun : kModePair;
+      // keys below 2^16 travel as 16-bit words from the first pass's output
+      // on (the stage API reads 32-bit keys)
+      col->narrow = !force_pair && a > 0 && col->maxval < 65536u;
       col->items = (uint32_t)items;
       // pair mode: values, then A, B, A, B; run mode: the cut writes B
       if (run_mode) col->final_buf = (a % 2) ? kBufA : kBufB;
@@ -596,6 +599,9 @@ __global__ __launch_bounds__(THREADS, WAH_SORT_WAVES_PER_EU) void k_onesweep(
       continue;
     }
     const uint32_t shift = col->shift[a];
+    // 16-bit keys: every pass writes them; the first reads the values (or
+    // the run cut's items) as 32-bit words
+    const bool narrow = col->narrow != 0u, src_narrow = narrow && !implicit && !(RUN && a == 0);
     if (t < 256) status_next[(uint64_t)tile * 256 + t] = 0;  // next launch's state
     const uint64_t wbase = base + (uint64_t)w * kWave * ITEMS + l;
     // A full tile runs a straight-line body (no per-item guards), so the
@@ -607,7 +613,12 @@ __global__ __launch_bounds__(THREADS, WAH_SORT_WAVES_PER_EU) void k_onesweep(
       uint32_t key[ITEMS], rank[ITEMS];
       const uint32_t nvalid =
           FULL ? (uint32_t)ITEMS * kWave : (uint32_t)(n > wbase ? n - wbase : 0);
-      {
+      if (src_narrow) {
+        const uint16_t* kp = reinterpret_cast<const uint16_t*>(src_keys + cd.off) + wbase;
+#pragma unroll
+        for (int i = 0; i < ITEMS; ++i)
+          key[i] = FULL || (uint32_t)(i * kWave) < nvalid ? (uint32_t)kp[i * kWave] : 0u;
+      } else {
         const uint32_t* kp = src_keys + cd.off + wbase;  // one base, immediate offsets
 #pragma unroll
         for (int i = 0; i < ITEMS; ++i)
@@ -712,6 +723,7 @@ __global__ __launch_bounds__(THREADS, WAH_SORT_WAVES_PER_EU) void k_onesweep(
       // destination of each slot this thread stores, kept for the rids (and
       // literal bits): the digit comes from the staged key
       uint32_t gq[ITEMS];
+      uint16_t* dk16 = reinterpret_cast<uint16_t*>(dk);
 #pragma unroll
       for (int j = 0; j < ITEMS; ++j) {
         const uint32_t q = (uint32_t)j * THREADS + t;
@@ -721,7 +733,8 @@ __global__ __launch_bounds__(THREADS, WAH_SORT_WAVES_PER_EU) void k_onesweep(
           const uint32_t g = s_glob[(k >> shift) & 255u] + q;
           bad |= g > last;
           gq[j] = g < last ? g : last;
-          dk[gq[j]] = k;
+          if (narrow) dk16[gq[j]] = (uint16_t)k;
+          else dk[gq[j]] = k;
         }
       }
       __syncthreads();
@@ -760,10 +773,14 @@ __global__ __launch_bounds__(THREADS, WAH_SORT_WAVES_PER_EU) void k_onesweep(
 // Selects the sorted arrays of one column (after its active passes); the
 // pointers address the column's first pair.
 struct SortedView {
-  const uint32_t* keys;
+  const uint32_t* keys;  // 32-bit keys, or 16-bit ones at the same start (narrow)
   const uint32_t* rids;
   const uint32_t* lits;  // run mode
   bool implicit;
+  bool narrow;
+  __device__ __forceinline__ uint32_t key(uint32_t i) const {
+    return narrow ? (uint32_t)reinterpret_cast<const uint16_t*>(keys)[i] : keys[i];
+  }
 };
 __device__ __forceinline__ SortedView sorted_view(const ColState* col, const Column& cd,
                                                   const uint32_t* values,
@@ -774,6 +791,7 @@ __device__ __forceinline__ SortedView sorted_view(const ColState* col, const Col
   SortedView v;
   const uint32_t f = col->final_buf;
   v.implicit = f == kBufValues;
+  v.narrow = col->narrow != 0u && f != kBufValues;
   v.keys = (f == kBufValues ? values : (f == kBufA ? keysA : keysB)) + cd.off;
   v.rids = (f == kBufA ? ridsA : ridsB) + cd.off;
   v.lits = (f == kBufA ? litsA : litsB) + cd.off;
@@ -811,7 +829,7 @@ __global__ __launch_bounds__(THREADS, tail_waves_per_eu(ITEMS)) void k_tail(
     uint32_t lanes, wahgpu_counts_dev* __restrict__ counts) {
   constexpr int TILE = THREADS * ITEMS;
   constexpr int WAVES = THREADS / kWave;
-  static_assert(ITEMS % 4 == 0 && ITEMS >= 8 && ITEMS <= 16, "16-byte loads; runs span <= 4 lanes");
+  static_assert(ITEMS % 8 == 0 && ITEMS <= 16, "16-byte loads (8 narrow keys); runs span <= 4 lanes");
   static_assert(2 * TILE < 65536, "tile word and key counts fit 16 bits");
   constexpr uint32_t kAll = (1u << ITEMS) - 1u;
   // staged words, one pad word per 16: lanes that write ~16 words apart
@@ -845,9 +863,13 @@ __global__ __launch_bounds__(THREADS, tail_waves_per_eu(ITEMS)) void k_tail(
       __syncthreads();  // s_tile is read by every wave before it is rewritten
       continue;
     }
+    // the tile's body, compiled once per key width (16-bit keys load 8 per
+    // 16 bytes); the column's width picks one
+    auto body = [&](auto narrow_tag) {
+    constexpr bool NARROW = decltype(narrow_tag)::value;
     const uint32_t rid_base = cd.rid_base;
-    const SortedView sv = sorted_view(&cs[c], cd, values, keysA, ridsA, keysB, ridsB, litsA,
-                                      litsB);
+    SortedView sv = sorted_view(&cs[c], cd, values, keysA, ridsA, keysB, ridsB, litsA, litsB);
+    sv.narrow = NARROW;
     const uint32_t j0 = base + (uint32_t)t * ITEMS;  // the lane's first item
     const uint32_t nv = n > j0 ? (n - j0 < (uint32_t)ITEMS ? n - j0 : (uint32_t)ITEMS) : 0u;
     const bool aligned = (((uintptr_t)sv.keys | (uintptr_t)sv.rids |
@@ -856,10 +878,24 @@ __global__ __launch_bounds__(THREADS, tail_waves_per_eu(ITEMS)) void k_tail(
     // ---- items of the lane: key, chid, literal bits (0 past the column)
     uint32_t key[ITEMS], ch[ITEMS], b[ITEMS];
     if (base + TILE <= n && aligned) {
+      if (NARROW) {  // 8 keys per 16-byte load
+#pragma unroll
+        for (int q = 0; q < ITEMS / 8; ++q) {
+          const uint4 k8 = reinterpret_cast<const uint4*>(reinterpret_cast<const uint16_t*>(sv.keys) + j0)[q];
+          const uint32_t w4[4] = {k8.x, k8.y, k8.z, k8.w};
+#pragma unroll
+          for (int h = 0; h < 4; ++h) {
+            key[8 * q + 2 * h] = w4[h] & 0xFFFFu;
+            key[8 * q + 2 * h + 1] = w4[h] >> 16;
+          }
+        }
+      }
 #pragma unroll
       for (int q = 0; q < ITEMS / 4; ++q) {
-        const uint4 k4 = reinterpret_cast<const uint4*>(sv.keys + j0)[q];
-        key[4 * q] = k4.x; key[4 * q + 1] = k4.y; key[4 * q + 2] = k4.z; key[4 * q + 3] = k4.w;
+        if (!NARROW) {
+          const uint4 k4 = reinterpret_cast<const uint4*>(sv.keys + j0)[q];
+          key[4 * q] = k4.x; key[4 * q + 1] = k4.y; key[4 * q + 2] = k4.z; key[4 * q + 3] = k4.w;
+        }
         if (sv.implicit) {
           const uint32_t r0 = rid_base + j0 + 4u * q;
           ch[4 * q] = r0; ch[4 * q + 1] = r0 + 1; ch[4 * q + 2] = r0 + 2; ch[4 * q + 3] = r0 + 3;
@@ -877,7 +913,7 @@ __global__ __launch_bounds__(THREADS, tail_waves_per_eu(ITEMS)) void k_tail(
       for (int i = 0; i < ITEMS; ++i) {
         const bool v = (uint32_t)i < nv;
         const uint32_t g = j0 + i;
-        key[i] = v ? sv.keys[g] : 0u;
+        key[i] = v ? sv.key(g) : 0u;
         ch[i] = v ? (sv.implicit ? rid_base + g : sv.rids[g]) : 0u;
         if (RUN) b[i] = v ? sv.lits[g] : 0u;
       }
@@ -895,7 +931,7 @@ __global__ __launch_bounds__(THREADS, tail_waves_per_eu(ITEMS)) void k_tail(
       const uint32_t gw = base + (uint32_t)(w * kWave * ITEMS);
       uint32_t k0 = 0, c0 = 0;
       if (gw > 0 && gw <= n) {
-        k0 = sv.keys[gw - 1];
+        k0 = sv.key(gw - 1);
         c0 = (sv.implicit ? rid_base + (gw - 1) : sv.rids[gw - 1]) / 31u;
       }
       pk = (uint32_t)__builtin_amdgcn_update_dpp((int)k0, (int)key[ITEMS - 1], 0x138, 0xF, 0xF, false);
@@ -950,7 +986,7 @@ __global__ __launch_bounds__(THREADS, tail_waves_per_eu(ITEMS)) void k_tail(
       uint32_t kk = 0, cc = 0, bb = 0;
       if (v) {
         const uint32_t r = sv.implicit ? rid_base + g : sv.rids[g];
-        kk = sv.keys[g];
+        kk = sv.key(g);
         cc = r / 31u;
         bb = RUN ? sv.lits[g] : 1u << (r - cc * 31u);
       }
@@ -1020,15 +1056,26 @@ __global__ __launch_bounds__(THREADS, tail_waves_per_eu(ITEMS)) void k_tail(
       // key starts: the values (the lane's keys read again, all at once:
       // the lines are still in L2) and the position of each key's first word
       uint32_t kv[ITEMS];
-      if (nv == (uint32_t)ITEMS && aligned) {
+      if (nv == (uint32_t)ITEMS && aligned && !NARROW) {
 #pragma unroll
         for (int q = 0; q < ITEMS / 4; ++q) {
           const uint4 k4 = reinterpret_cast<const uint4*>(sv.keys + j0)[q];
           kv[4 * q] = k4.x; kv[4 * q + 1] = k4.y; kv[4 * q + 2] = k4.z; kv[4 * q + 3] = k4.w;
         }
+      } else if (nv == (uint32_t)ITEMS && aligned) {
+#pragma unroll
+        for (int q = 0; q < ITEMS / 8; ++q) {
+          const uint4 k8 = reinterpret_cast<const uint4*>(reinterpret_cast<const uint16_t*>(sv.keys) + j0)[q];
+          const uint32_t w4[4] = {k8.x, k8.y, k8.z, k8.w};
+#pragma unroll
+          for (int h = 0; h < 4; ++h) {
+            kv[8 * q + 2 * h] = w4[h] & 0xFFFFu;
+            kv[8 * q + 2 * h + 1] = w4[h] >> 16;
+          }
+        }
       } else {
 #pragma unroll
-        for (int i = 0; i < ITEMS; ++i) kv[i] = (uint32_t)i < nv ? sv.keys[j0 + i] : 0u;
+        for (int i = 0; i < ITEMS; ++i) kv[i] = (uint32_t)i < nv ? sv.key(j0 + i) : 0u;
       }
       bool bad = false;
 #pragma unroll
@@ -1056,6 +1103,9 @@ __global__ __launch_bounds__(THREADS, tail_waves_per_eu(ITEMS)) void k_tail(
       counts[cd.index].length = (uint64_t)Wpre + Wt;
       counts[cd.index].keycnt = (uint64_t)Kpre + (tot >> 16);
     }
+    };
+    if (cs[c].narrow) body(std::true_type{});
+    else body(std::false_type{});
     // the next tile's first barrier (in next_tile) orders these LDS reads
     // before its writes
   }
@@ -1140,7 +1190,7 @@ __global__ __launch_bounds__(kBlock) void k_sort_finalize(
   if (sv.keys == out_keys && !sv.implicit) return;
   for (uint64_t i = (uint64_t)blockIdx.x * kBlock + threadIdx.x; i < cd.n;
        i += (uint64_t)gridDim.x * kBlock) {
-    out_keys[i] = sv.keys[i];
+    out_keys[i] = sv.key((uint32_t)i);
     out_rids[i] = sv.implicit ? cd.rid_base + (uint32_t)i : sv.rids[i];
   }
 }
@@ -1480,11 +1530,11 @@ void launch_tail(const BuildArgs& a, hipStream_t s, LaunchHook* hook) {
   const uint32_t rgrid =
       cap_grid(a.tail_tiles_cap, WAH_STATIC_TILES ? a.tail_tiles_cap
                                                   : a.max_resident / 4 * kRunTailBlocksPerCU);
+  const uint32_t rlanes = WAH_TAIL_LANES ? WAH_TAIL_LANES : tile_lanes(rgrid);
   hipLaunchKernelGGL((k_tail<kRunTailThreads, kRunTailItems, true>), dim3(rgrid),
                      dim3(kRunTailThreads), 0, s, a.desc, a.cs, a.hdr, a.values, a.keysA, a.ridsA,
                      a.keysB, a.ridsB, a.litsA, a.litsB, a.out_keys, a.out_offsets, a.out_words,
-                     a.tail_status, a.tile_entries, WAH_TAIL_LANES ? WAH_TAIL_LANES : tile_lanes(rgrid),
-                     a.counts);
+                     a.tail_status, a.tile_entries, rlanes, a.counts);
   if (hook) hook->mark(s);
   uint64_t lg = (a.n_cap + kBlock - 1) / kBlock;
   hipLaunchKernelGGL(k_finish, dim3(cap_grid(lg > 1024 ? 1024u : (uint32_t)lg, 1024)),

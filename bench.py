@@ -10,8 +10,12 @@ Beside it:
   roofline      the dominant kernel (the radix passes, k_onesweep) timed with
                 HIP events on the build's stream over the timed steps;
                 achieved = its algorithmic bytes per launch / its mean launch
-                time (pass 1: 4n read + 8n written; later passes 16n)
-  build_roofline the same for the whole build: B_alg = 16nP + 8n + 4L + 12K
+                time.  Keys below 2^16 travel as 2-byte words (kb = 2, else
+                4): pass 1 reads 4n and writes (kb + 4)n; later passes read
+                and write (kb + 4)n each
+  build_roofline the same for the whole build: B_alg = (2kb + 8)nP + 8n + 4L
+                + 12K (histogram 4n, tail (kb + 4)n read, words 4L, keys,
+                offsets and lengths 12K)
   cpu_baseline  the oracle (a C restatement of the reference's serial `cpu`
                 program, one thread) on a 2^26-value sample of the same
                 distribution, timed on this host
@@ -273,16 +277,17 @@ def main():
     pass_ms = [group_ms[1 + p] / calls for p in range(4)]
     # the P active passes are the P slowest (inactive ones exit at once)
     active = sorted(sorted(range(4), key=lambda p: -pass_ms[p])[:P])
+    kb = 2.0 if counts.bound < 65536 and P > 0 else 4.0     # bytes per key between passes
     if counts.mode == 0:
-        pass_bytes = [12.0 * n] + [16.0 * n] * (P - 1)     # per active launch, in order
-    else:  # run mode: every launch moves 12-byte run items (at least k of them) in and out
-        pass_bytes = [24.0 * counts.k] * P
+        pass_bytes = [(8.0 + kb) * n] + [(2.0 * kb + 8.0) * n] * (P - 1)  # per active launch
+    else:  # run mode: (key, rid, literal) items, at least k of them, in and out
+        pass_bytes = [(20.0 + kb) * counts.k] + [(2.0 * kb + 16.0) * counts.k] * (P - 1)
     sort_ms = sum(pass_ms[p] for p in active)
     per_launch_bytes = sum(pass_bytes) / P
     per_launch_ms = sort_ms / P
     achieved = per_launch_bytes / (per_launch_ms * 1e-3) / 1e9
     traffic = read_traffic() if args.column == "uniform" and n == N_VALUES else None
-    b_alg = 16.0 * n * P + 8.0 * n + 4.0 * L + 12.0 * K
+    b_alg = (2.0 * kb + 8.0) * n * P + 8.0 * n + 4.0 * L + 12.0 * K
     build_ms_events = sum(group_ms) / calls
 
     result = {

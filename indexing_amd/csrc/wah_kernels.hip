@@ -466,31 +466,47 @@ __global__ __launch_bounds__(kBlock, 8) void k_runcut(
         if (t == 0) atomicOr(&hdr->fault, 2u);
         return;  // block-uniform: tile_runs disagree with the items
       }
-      // three rounds through LDS: values, rids, literal bits
-#pragma unroll 1
-      for (int round = 0; round < 3; ++round) {
+      // the item of start i: round 0 its value, 1 its rid, 2 its literal bits
+      auto item = [&](int round, int i) -> uint32_t {
+        if (round == 0) return v[i];
+        if (round == 1) return cd.rid_base + j0 + i;
+        const uint32_t rest = stops >> (i + 1);
+        const uint32_t len = rest ? (uint32_t)__builtin_ctz(rest) + 1u : (uint32_t)(R - i) + cont;
+        const uint32_t mi = m0 + i >= 31u ? m0 + i - 31u : m0 + i;
+        return ((len >= 32u ? 0u : (1u << len)) - 1u) << mi;
+      };
+      if (3 * tot <= (uint32_t)kCutStep) {
+        // few runs: the three arrays staged together, one round
         uint32_t k = k0;
 #pragma unroll
         for (int i = 0; i < R; ++i) {
           if ((sm >> i) & 1u) {
-            uint32_t x;
-            if (round == 0) {
-              x = v[i];
-            } else if (round == 1) {
-              x = cd.rid_base + j0 + i;
-            } else {
-              const uint32_t rest = stops >> (i + 1);
-              const uint32_t len = rest ? (uint32_t)__builtin_ctz(rest) + 1u : (uint32_t)(R - i) + cont;
-              const uint32_t mi = m0 + i >= 31u ? m0 + i - 31u : m0 + i;
-              x = ((len >= 32u ? 0u : (1u << len)) - 1u) << mi;
-            }
-            s_stage[k++] = x;
+            s_stage[k] = item(0, i);
+            s_stage[tot + k] = item(1, i);
+            s_stage[2 * tot + k] = item(2, i);
+            ++k;
           }
         }
         __syncthreads();
-        uint32_t* dst = (round == 0 ? keysB : round == 1 ? ridsB : litsB) + cd.off + off;
-        for (uint32_t j = t; j < tot; j += kBlock) dst[j] = s_stage[j];
+        for (uint32_t j = t; j < 3 * tot; j += kBlock) {
+          const uint32_t r = j / tot, q = j - r * tot;  // tot > 0 here
+          uint32_t* dst = (r == 0 ? keysB : r == 1 ? ridsB : litsB) + cd.off + off;
+          dst[q] = s_stage[j];
+        }
         __syncthreads();
+      } else {
+        // three rounds through LDS: values, rids, literal bits
+#pragma unroll 1
+        for (int round = 0; round < 3; ++round) {
+          uint32_t k = k0;
+#pragma unroll
+          for (int i = 0; i < R; ++i)
+            if ((sm >> i) & 1u) s_stage[k++] = item(round, i);
+          __syncthreads();
+          uint32_t* dst = (round == 0 ? keysB : round == 1 ? ridsB : litsB) + cd.off + off;
+          for (uint32_t j = t; j < tot; j += kBlock) dst[j] = s_stage[j];
+          __syncthreads();
+        }
       }
       off += tot;
     }

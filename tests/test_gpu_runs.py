@@ -97,3 +97,43 @@ def test_batch_mixes_modes(ctx):
     assert {c.mode for c in counts} == {0, 1}
     for i, c in enumerate(cols):
         assert_same(got[i], oracle.build(c, rid_base=rid_bases[i]), f"column {i}")
+
+
+def key_width_columns():
+    """Columns on both sides of the 16-bit key boundary (keys below 2^16 move
+    between radix passes as 2-byte words), in pair and run mode."""
+    rng = np.random.default_rng(21)
+    top16 = rng.integers(0, 65536, 90000).astype(np.uint32)
+    top16[rng.integers(0, top16.size)] = 65535            # max 65535: 2-byte keys
+    over = top16.copy()
+    over[rng.integers(0, over.size)] = 65536              # max 65536: 4-byte keys
+    runs16 = runs_column(70000, 40, 65536, 22, 1 << 16)
+    runs16[-1] = 65535
+    runs17 = runs16.copy()
+    runs17[100] = 1 << 16
+    byte_keys = rng.integers(0, 256, 50000).astype(np.uint32)  # one active pass
+    return [("max_65535", top16), ("max_65536", over), ("runs_max_65535", runs16),
+            ("runs_max_65536", runs17), ("one_pass", byte_keys)]
+
+
+@pytest.mark.parametrize("name,values", key_width_columns(), ids=[c[0] for c in key_width_columns()])
+def test_key_width_boundary(ctx, name, values):
+    print(f"[width] {name} n={values.size} max={int(values.max())}", flush=True)
+    got = ctx.build(values, rid_base=3)
+    assert got.bound == int(values.max())
+    assert_same(got, oracle.build(values, rid_base=3), name)
+
+
+def test_key_widths_mixed_in_one_batch(ctx):
+    cols = [v for _, v in key_width_columns()]
+    rng = np.random.default_rng(23)
+    cols += [rng.integers(0, 1 << 32, 20000, dtype=np.uint64).astype(np.uint32),
+             rng.integers(0, 1000, 30001).astype(np.uint32)]
+    buf, layout = pack(cols, gap=3, align=2)
+    print(f"[width] batch of {len(cols)}", flush=True)
+    out = ctx.build_batch_device(dev(buf), layout)
+    counts = ctx.read_counts_batch(out[4], len(cols))
+    got = per_column(out, counts, layout)
+    for i, c in enumerate(cols):
+        assert counts[i].fault == 0
+        assert_same(got[i], oracle.build(c), f"column {i}")

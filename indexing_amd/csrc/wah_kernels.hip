@@ -1,16 +1,21 @@
 // wah_kernels.hip -- gfx950 kernels of the WAH bitmap-index build.
 //
-// Fused build (what wahgpu_build runs), for a column of n uint32 values:
-//   k_fill32      zeroes the sort header (histogram, counters)
+// Fused build (what wahgpu_build and the batched builds run), for columns of
+// uint32 values:
+//   k_begin       zeroes the header and per-column state (and writes a single
+//                 column's descriptor)
 //   k_histogram   one read of the values: histograms of all four 8-bit
-//                 digits and the maximum, zeroing of look-back state
-//   k_plan        digit offsets; passes whose digit is the same for every
-//                 value are switched off (on the device, no host sync)
-//   k_onesweep x4 stable LSD radix pass per active digit: wave64 ballot
-//                 ranking, LDS staging, decoupled look-back over tiles
-//   k_tail        sorted (value, rid) -> chunk ids, OR-merged literals, fills,
+//                 digits, repeat rows, run counts per radix tile, the maximum;
+//                 zeroing of look-back state
+//   k_plan        per column: sort mode (pairs or runs), key width, digit
+//                 offsets; passes whose digit is the same for every value are
+//                 switched off (on the device, no host sync)
+//   k_runcut      run mode: rows -> one (value, rid, literal bits) item per run
+//   k_onesweep x4 stable LSD radix pass per active digit: LDS fetch-add (or
+//                 ballot) ranking, LDS staging, decoupled look-back over tiles
+//   k_tail        sorted items -> chunk ids, OR-merged literals, fills,
 //                 zero-fill removal, word positions, keys and offsets
-//   k_lengths     lengths[i] = offsets[i+1] - offsets[i]
+//   k_finish      per column: k, lengths[i] = offsets[i+1] - offsets[i], fault
 // Staged kernels (what `phases` runs, one per reference stage) and the scan /
 // compaction primitives reuse the same look-back scheme.
 #include <hip/hip_runtime.h>
@@ -337,15 +342,6 @@ __global__ __launch_bounds__(kBlock) void k_plan(const DescHead* __restrict__ de
 }
 
 // ---------------------------------------------------------------------------
-// Launch `a` of the stable LSD radix sort runs each column's a-th active pass
-// (Onesweep-style): launch 0 reads the values (rids implicit) into buffer A,
-// then A -> B, B -> A, A -> B.  Tile = THREADS x ITEMS keys of one column;
-// wave w ranks keys [w*64*ITEMS, +64*ITEMS) of the tile in (item, lane)
-// order, which is the input order, so equal digits keep their relative order
-// (stability).  Keys and then rids are staged in one LDS buffer in digit
-// order and written out in runs per digit.
-// ---------------------------------------------------------------------------
-// ---------------------------------------------------------------------------
 // Run mode, before the radix launches: cuts each run-mode column into runs
 // (a row starts one unless it repeats its predecessor's value inside the
 // same 31-row chunk) and writes one item per run, in row order, to buffer B:
@@ -557,6 +553,16 @@ __global__ __launch_bounds__(256) void k_rank_selftest(uint32_t* ok) {
   if (!good) atomicAnd(ok, 0u);
 }
 
+// ---------------------------------------------------------------------------
+// Launch `a` of the stable LSD radix sort runs each column's a-th active pass
+// (Onesweep-style): launch 0 reads the values (rids implicit) into buffer A,
+// then A -> B, B -> A, A -> B.  Tile = THREADS x ITEMS keys of one column;
+// wave w ranks keys [w*64*ITEMS, +64*ITEMS) of the tile in (item, lane)
+// order, which is the input order, so equal digits keep their relative order
+// (stability).  Keys and then rids are staged in one LDS buffer in digit
+// order and written out in runs per digit.  Run mode moves (key, rid,
+// literal bits) items the run cut wrote to B: launch 0 reads B.
+// ---------------------------------------------------------------------------
 template <int THREADS, int ITEMS, bool RUN, int RANK>
 __global__ __launch_bounds__(THREADS, WAH_SORT_WAVES_PER_EU) void k_onesweep(
     uint32_t a, const uint32_t* __restrict__ values, uint32_t* keysA, uint32_t* ridsA,

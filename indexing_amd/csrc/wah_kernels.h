@@ -52,10 +52,10 @@ constexpr int kTailTile = kTailThreads * kTailItems;  // sorted pairs per tail t
 #endif
 constexpr int kTailWavesPerEU = WAH_TAIL_WAVES_PER_EU;
 constexpr int kTailBlocksPerCU = kTailWavesPerEU * 4 * 64 / kTailThreads;
-// Run-mode tail: the same 8192-item tiles as 512 threads x 16 items at 4
-// waves per SIMD (128 VGPRs), two blocks per CU.
-constexpr int kRunTailThreads = 512;
-constexpr int kRunTailItems = kTailTile / kRunTailThreads;
+// Run-mode tail: the same tiles as 16 items per thread (512 threads for the
+// 8192-item tile) at 4 waves per SIMD (128 VGPRs), two blocks per CU.
+constexpr int kRunTailItems = 16;
+constexpr int kRunTailThreads = kTailTile / kRunTailItems;
 constexpr int kRunTailBlocksPerCU = 4 * 4 * 64 / kRunTailThreads;
 static_assert(kRunTailThreads * kRunTailItems == kTailTile, "run tail tiles = pair tail tiles");
 // waves per SIMD a tail instantiation is compiled for

@@ -318,10 +318,12 @@ __device__ uint64_t lookback64(uint64_t* status, uint32_t tile, uint64_t agg, ui
 }
 
 // Look-back for commutative sums, run by a whole wave: each round reads the
-// statuses of the 64 preceding tiles at once and stops at the nearest
-// inclusive prefix.  Returns the exclusive prefix in every lane.
-// Tile `first` opens a segment (a column): it publishes its prefix at once and
-// nothing below it is read.
+// statuses of the K x 64 preceding tiles at once (K per lane, nearest first)
+// and stops at the nearest inclusive prefix; slots complete up to a tile
+// without a status are kept and the round resumes from there.  Returns the
+// exclusive prefix in every lane.  Tile `first` opens a segment
+// (a column): it publishes its prefix at once and nothing below it is read.
+template <int K>
 __device__ uint64_t lookback_wave_sum(uint64_t* status, uint32_t tile, uint32_t first,
                                       uint64_t agg, uint32_t* fault) {
   constexpr uint64_t kMask = (1ull << 62) - 1;
@@ -330,22 +332,38 @@ __device__ uint64_t lookback_wave_sum(uint64_t* status, uint32_t tile, uint32_t 
     st_agent(&status[tile], ((uint64_t)(tile == first ? kFlagPrefix : kFlagAgg) << 62) | (agg & kMask));
   if (tile == first) return 0;
   uint64_t acc = 0;
-  int64_t base = (int64_t)tile - 1;
+  int64_t base = (int64_t)tile - 1;  // nearest tile not summed yet
   const int64_t lo = first;
   uint32_t spins = 0;
   for (;;) {
-    const int64_t idx = base - (int64_t)lane;
-    const uint64_t s = idx >= lo ? ld_agent(&status[idx]) : ((uint64_t)kFlagPrefix << 62);
-    const uint32_t f = (uint32_t)(s >> 62);
-    const uint64_t pmask = ballot(f == kFlagPrefix);
-    const uint64_t rmask = ballot(f != 0);
-    const uint64_t need = pmask ? (((pmask & (~pmask + 1)) << 1) - 1) : ~0ull;
-    if ((rmask & need) == need) {
-      acc += wave_sum64(((need >> lane) & 1u) ? (s & kMask) : 0);
-      if (pmask) break;
-      base -= kWave;
-      continue;
+    uint64_t s[K];
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+      const int64_t idx = base - (int64_t)(k * kWave) - (int64_t)lane;
+      s[k] = idx >= lo ? ld_agent(&status[idx]) : ((uint64_t)kFlagPrefix << 62);
     }
+    uint64_t take = 0;
+    int done = 0;  // slots summed in full
+    bool found = false, stall = false;
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+      if (found || stall) continue;  // wave-uniform
+      const uint32_t f = (uint32_t)(s[k] >> 62);
+      const uint64_t pmask = ballot(f == kFlagPrefix);
+      const uint64_t rmask = ballot(f != 0);
+      const uint64_t need = pmask ? (((pmask & (~pmask + 1)) << 1) - 1) : ~0ull;
+      if ((rmask & need) != need) {
+        stall = true;
+        continue;
+      }
+      take += ((need >> lane) & 1u) ? (s[k] & kMask) : 0;
+      if (pmask) found = true;
+      else ++done;
+    }
+    acc += wave_sum64(take);
+    if (found) break;
+    base -= (int64_t)done * kWave;
+    if (!stall) continue;
     if (++spins >= kSpinLimit) {
       if (lane == 0) atomicOr(fault, 1u);
       break;

@@ -47,6 +47,13 @@ constexpr int kTailTile = kTailThreads * kTailItems;  // sorted pairs per tail t
 #ifndef WAH_TAIL_LANES  // tile counters of the tail (0: as many as the grid allows, up to 8)
 #define WAH_TAIL_LANES 0
 #endif
+// Tail look-back window: 64 x this many tiles per round trip.  The wait is
+// for the aggregates of the tiles just before (their loads finish at
+// different times), not for the walk, so one slot is best (1.32 ms on 2^28
+// values against 1.38 / 1.41 / 1.51 ms for 2 / 4 / 8).
+#ifndef WAH_TAIL_LOOKBACK_SLOTS
+#define WAH_TAIL_LOOKBACK_SLOTS 1
+#endif
 #ifndef WAH_TAIL_WAVES_PER_EU
 #define WAH_TAIL_WAVES_PER_EU 8
 #endif

@@ -1046,7 +1046,8 @@ __global__ __launch_bounds__(THREADS, tail_waves_per_eu(ITEMS)) void k_tail(
     const uint32_t lpos = woff + inc - packed;  // (keys << 16 | words) before the lane
     if (w == 0) {
       const uint64_t tile_agg = ((uint64_t)(tot >> 16) << 31) | (tot & 0xFFFFu);
-      const uint64_t pre = lookback_wave_sum(status, tile, first_tile, tile_agg, fault);
+      const uint64_t pre = lookback_wave_sum<WAH_TAIL_LOOKBACK_SLOTS>(status, tile, first_tile,
+                                                                      tile_agg, fault);
       const uint32_t e = readlane(wave_incl_add(l < WAVES ? s_went[l] : 0u), kWave - 1);
       if (l == 0) {
         s_prefix = pre;

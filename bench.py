@@ -10,12 +10,16 @@ Beside it:
   roofline      the dominant kernel (the radix passes, k_onesweep) timed with
                 HIP events on the build's stream over the timed steps;
                 achieved = its algorithmic bytes per launch / its mean launch
-                time.  Keys below 2^16 travel as 2-byte words (kb = 2, else
-                4): pass 1 reads 4n and writes (kb + 4)n; later passes read
-                and write (kb + 4)n each
-  build_roofline the same for the whole build: B_alg = (2kb + 8)nP + 8n + 4L
-                + 12K (histogram 4n, tail (kb + 4)n read, words 4L, keys,
-                offsets and lengths 12K)
+                time, with the canonical pipeline's bytes: (value, rid) pairs
+                of 8 bytes, pass 1 reads 4n and writes 8n, later passes read
+                and write 8n each.  moved_* is the same for the bytes this
+                build moves: keys below 2^16 travel as 2-byte words (kb = 2,
+                else 4), so pass 1 writes (kb + 4)n and later passes read and
+                write (kb + 4)n each
+  build_roofline the same for the whole build: B_alg = 16nP + 8n + 4L + 12K
+                (histogram 4n, passes, tail 8n read, words 4L, keys, offsets
+                and lengths 12K); moved_bytes = (2kb + 8)nP + 8n + 4L + 12K
+                with the tail reading (kb + 4)n
   cpu_baseline  the oracle (a C restatement of the reference's serial `cpu`
                 program, one thread) on a 2^26-value sample of the same
                 distribution, timed on this host
@@ -277,17 +281,22 @@ def main():
     pass_ms = [group_ms[1 + p] / calls for p in range(4)]
     # the P active passes are the P slowest (inactive ones exit at once)
     active = sorted(sorted(range(4), key=lambda p: -pass_ms[p])[:P])
+    # canonical bytes per active launch: 8-byte (value, rid) pairs over n rows
+    alg_pass_bytes = [12.0 * n] + [16.0 * n] * (P - 1)
     kb = 2.0 if counts.bound < 65536 and P > 0 else 4.0     # bytes per key between passes
     if counts.mode == 0:
-        pass_bytes = [(8.0 + kb) * n] + [(2.0 * kb + 8.0) * n] * (P - 1)  # per active launch
+        pass_bytes = [(8.0 + kb) * n] + [(2.0 * kb + 8.0) * n] * (P - 1)  # moved per launch
     else:  # run mode: (key, rid, literal) items, at least k of them, in and out
         pass_bytes = [(20.0 + kb) * counts.k] + [(2.0 * kb + 16.0) * counts.k] * (P - 1)
     sort_ms = sum(pass_ms[p] for p in active)
-    per_launch_bytes = sum(pass_bytes) / P
-    per_launch_ms = sort_ms / P
-    achieved = per_launch_bytes / (per_launch_ms * 1e-3) / 1e9
+    per_launch_alg = sum(alg_pass_bytes) / P if P else 0.0
+    per_launch_bytes = sum(pass_bytes) / P if P else 0.0
+    per_launch_ms = sort_ms / P if P else 0.0
+    achieved = per_launch_alg / (per_launch_ms * 1e-3) / 1e9 if P else 0.0
+    moved = per_launch_bytes / (per_launch_ms * 1e-3) / 1e9 if P else 0.0
     traffic = read_traffic() if args.column == "uniform" and n == N_VALUES else None
-    b_alg = (2.0 * kb + 8.0) * n * P + 8.0 * n + 4.0 * L + 12.0 * K
+    b_alg = 16.0 * n * P + 8.0 * n + 4.0 * L + 12.0 * K
+    b_moved = (2.0 * kb + 8.0) * n * P + 8.0 * n + 4.0 * L + 12.0 * K
     build_ms_events = sum(group_ms) / calls
 
     result = {
@@ -313,12 +322,18 @@ def main():
                      "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": traffic,
-                     "alg_bytes_per_launch": per_launch_bytes,
-                     "mean_launch_ms": round(per_launch_ms, 4)},
+                     "alg_bytes_per_launch": per_launch_alg,
+                     "mean_launch_ms": round(per_launch_ms, 4),
+                     "moved_bytes_per_launch": per_launch_bytes,
+                     "moved_achieved": round(moved, 1),
+                     "moved_frac": round(moved / HBM_PEAK_GBS, 4)},
         "build_roofline": {"alg_bytes": b_alg, "ms": round(build_ms_events, 4),
                            "achieved": round(b_alg / (build_ms_events * 1e-3) / 1e9, 1),
                            "frac": round(b_alg / (build_ms_events * 1e-3) / 1e9 / HBM_PEAK_GBS,
-                                         4)},
+                                         4),
+                           "moved_bytes": b_moved,
+                           "moved_frac": round(b_moved / (build_ms_events * 1e-3) / 1e9
+                                               / HBM_PEAK_GBS, 4)},
         "kernel_ms": {name: round(g / calls, 4) for name, g in zip(
             ["histogram", "pass0", "pass1", "pass2", "pass3", "tail", "lengths"], group_ms)},
     }

@@ -19,7 +19,8 @@ Beside it:
   build_roofline the same for the whole build: B_alg = 16nP + 8n + 4L + 12K
                 (histogram 4n, passes, tail 8n read, words 4L, keys, offsets
                 and lengths 12K); moved_bytes = (2kb + 8)nP + 8n + 4L + 12K
-                with the tail reading (kb + 4)n
+                with the tail reading (kb + 4)n (run mode: the k run items
+                through the cut, the passes and the tail)
   cpu_baseline  the oracle (a C restatement of the reference's serial `cpu`
                 program, one thread) on a 2^26-value sample of the same
                 distribution, timed on this host
@@ -296,7 +297,11 @@ def main():
     moved = per_launch_bytes / (per_launch_ms * 1e-3) / 1e9 if P else 0.0
     traffic = read_traffic() if args.column == "uniform" and n == N_VALUES else None
     b_alg = 16.0 * n * P + 8.0 * n + 4.0 * L + 12.0 * K
-    b_moved = (2.0 * kb + 8.0) * n * P + 8.0 * n + 4.0 * L + 12.0 * K
+    if counts.mode == 0:  # histogram 4n, passes, tail (kb + 4)n, outputs
+        b_moved = (2.0 * kb + 8.0) * n * P + 8.0 * n + 4.0 * L + 12.0 * K
+    else:  # histogram 4n, run cut 4n + 12k, passes, tail (kb + 8)k, outputs
+        b_moved = (8.0 * n + 12.0 * counts.k + sum(pass_bytes) + (kb + 8.0) * counts.k
+                   + 4.0 * L + 12.0 * K)
     build_ms_events = sum(group_ms) / calls
 
     result = {

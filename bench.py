@@ -64,6 +64,35 @@ def read_traffic():
     return None
 
 
+def byte_model(n, k, L, K, P, bound, mode):
+    """Bytes of one build of a column: n rows, k (value, chunk) entries, L WAH
+    words, K keys, P active radix passes, maximum value `bound`, sort mode
+    (0 pairs, 1 runs).
+
+    alg_pass / b_alg are the canonical pipeline's bytes: 8-byte (value, rid)
+    pairs, radix pass 1 reads 4n values and writes 8n, each later pass reads
+    and writes 8n; the whole build is B_alg = 16nP + 8n + 4L + 12K (histogram
+    4n, passes, the tail's 8n read, words 4L, keys, offsets and lengths 12K).
+
+    moved_pass / b_moved are the bytes this build moves: keys below 2^16
+    travel between passes as 2-byte words (kb = 2, else 4); a run-mode column
+    sorts one (key, rid, literal bits) item per run, at least k of them, cut
+    from a second read of the values.
+    """
+    alg_pass = [12.0 * n] + [16.0 * n] * (P - 1) if P else []
+    kb = 2.0 if bound < 65536 and P > 0 else 4.0
+    if mode == 0:
+        moved_pass = [(8.0 + kb) * n] + [(2.0 * kb + 8.0) * n] * (P - 1) if P else []
+        b_moved = (2.0 * kb + 8.0) * n * P + 8.0 * n + 4.0 * L + 12.0 * K
+    else:
+        moved_pass = [(20.0 + kb) * k] + [(2.0 * kb + 16.0) * k] * (P - 1) if P else []
+        # histogram 4n, run cut 4n + 12k, passes, tail (kb + 8)k, outputs
+        b_moved = 8.0 * n + 12.0 * k + sum(moved_pass) + (kb + 8.0) * k + 4.0 * L + 12.0 * K
+    b_alg = 16.0 * n * P + 8.0 * n + 4.0 * L + 12.0 * K
+    return {"alg_pass": alg_pass, "moved_pass": moved_pass, "b_alg": b_alg, "b_moved": b_moved,
+            "kb": kb}
+
+
 def progress(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
@@ -282,26 +311,15 @@ def main():
     pass_ms = [group_ms[1 + p] / calls for p in range(4)]
     # the P active passes are the P slowest (inactive ones exit at once)
     active = sorted(sorted(range(4), key=lambda p: -pass_ms[p])[:P])
-    # canonical bytes per active launch: 8-byte (value, rid) pairs over n rows
-    alg_pass_bytes = [12.0 * n] + [16.0 * n] * (P - 1)
-    kb = 2.0 if counts.bound < 65536 and P > 0 else 4.0     # bytes per key between passes
-    if counts.mode == 0:
-        pass_bytes = [(8.0 + kb) * n] + [(2.0 * kb + 8.0) * n] * (P - 1)  # moved per launch
-    else:  # run mode: (key, rid, literal) items, at least k of them, in and out
-        pass_bytes = [(20.0 + kb) * counts.k] + [(2.0 * kb + 16.0) * counts.k] * (P - 1)
+    bm = byte_model(n, int(counts.k), L, K, P, int(counts.bound), int(counts.mode))
     sort_ms = sum(pass_ms[p] for p in active)
-    per_launch_alg = sum(alg_pass_bytes) / P if P else 0.0
-    per_launch_bytes = sum(pass_bytes) / P if P else 0.0
+    per_launch_alg = sum(bm["alg_pass"]) / P if P else 0.0
+    per_launch_bytes = sum(bm["moved_pass"]) / P if P else 0.0
     per_launch_ms = sort_ms / P if P else 0.0
     achieved = per_launch_alg / (per_launch_ms * 1e-3) / 1e9 if P else 0.0
     moved = per_launch_bytes / (per_launch_ms * 1e-3) / 1e9 if P else 0.0
     traffic = read_traffic() if args.column == "uniform" and n == N_VALUES else None
-    b_alg = 16.0 * n * P + 8.0 * n + 4.0 * L + 12.0 * K
-    if counts.mode == 0:  # histogram 4n, passes, tail (kb + 4)n, outputs
-        b_moved = (2.0 * kb + 8.0) * n * P + 8.0 * n + 4.0 * L + 12.0 * K
-    else:  # histogram 4n, run cut 4n + 12k, passes, tail (kb + 8)k, outputs
-        b_moved = (8.0 * n + 12.0 * counts.k + sum(pass_bytes) + (kb + 8.0) * counts.k
-                   + 4.0 * L + 12.0 * K)
+    b_alg, b_moved = bm["b_alg"], bm["b_moved"]
     build_ms_events = sum(group_ms) / calls
 
     result = {

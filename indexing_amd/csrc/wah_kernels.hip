@@ -628,8 +628,13 @@ __global__ __launch_bounds__(THREADS, WAH_SORT_WAVES_PER_EU) void k_onesweep(
     const uint64_t wbase = base + (uint64_t)w * kWave * ITEMS + l;
     // A full tile runs a straight-line body (no per-item guards), so the
     // compiler can wait for the rid loads without draining the key stores.
-    auto sweep = [&](auto full_tag) {
+    // PACKED: launch 0 of a pair-mode column with keys < 2^16.  Its rids are
+    // implicit (rid_base + column row), so one LDS word carries both the key
+    // (high half) and the row's tile-local index (low half): one staging
+    // round moves keys and rids.
+    auto sweep = [&](auto full_tag, auto packed_tag) {
       constexpr bool FULL = decltype(full_tag)::value;
+      constexpr bool PACKED = decltype(packed_tag)::value;
       // Keys only for now: the rids are loaded once the keys are staged, so
       // they never occupy registers during ranking and look-back.
       uint32_t key[ITEMS], rank[ITEMS];
@@ -711,6 +716,38 @@ __global__ __launch_bounds__(THREADS, WAH_SORT_WAVES_PER_EU) void k_onesweep(
         for (int i = 0; i < WAVES; ++i) s_wcnt[i][t] += ex;
       }
       __syncthreads();
+      if constexpr (PACKED) {
+        const uint32_t row0 = (uint32_t)(wbase - base);  // tile-local index of item 0
+#pragma unroll
+        for (int i = 0; i < ITEMS; ++i) {
+          if (FULL || ((part >> i) & 1u)) {
+            const uint32_t d = (key[i] >> shift) & 255u;
+            s_stage[s_wcnt[w][d] + rank[i]] = key[i] << 16 | (row0 + (uint32_t)(i * kWave));
+          }
+        }
+        __syncthreads();
+        const uint32_t tile_n = FULL ? (uint32_t)TILE : s_tilen;
+        uint16_t* dk16 = reinterpret_cast<uint16_t*>(dst_keys + cd.off);
+        uint32_t* dr = dst_rids + cd.off;
+        const uint32_t last = (uint32_t)(n_out - 1);  // stores stay in the column
+        const uint32_t rid0 = cd.rid_base + (uint32_t)base;
+        bool bad = false;
+#pragma unroll
+        for (int j = 0; j < ITEMS; ++j) {
+          const uint32_t q = (uint32_t)j * THREADS + t;
+          if (FULL || q < tile_n) {
+            const uint32_t x = s_stage[q];
+            const uint32_t k = x >> 16;
+            const uint32_t g = s_glob[(k >> shift) & 255u] + q;
+            bad |= g > last;
+            const uint32_t gc = g < last ? g : last;
+            dk16[gc] = (uint16_t)k;
+            dr[gc] = rid0 + (x & 0xFFFFu);
+          }
+        }
+        if (ballot(bad) && l == 0) atomicOr(fault, 2u);  // a scatter index past the column
+        return;
+      }
 #pragma unroll
       for (int i = 0; i < ITEMS; ++i) {
         if (FULL || ((part >> i) & 1u)) {
@@ -786,8 +823,16 @@ __global__ __launch_bounds__(THREADS, WAH_SORT_WAVES_PER_EU) void k_onesweep(
       }
       if (ballot(bad) && l == 0) atomicOr(fault, 2u);  // a scatter index past the column
     };
-    if (base + TILE <= n) sweep(std::true_type{});
-    else sweep(std::false_type{});
+    auto dispatch = [&](auto packed_tag) {
+      if (base + TILE <= n) sweep(std::true_type{}, packed_tag);
+      else sweep(std::false_type{}, packed_tag);
+    };
+    if constexpr (!RUN) {
+      if (implicit && narrow) dispatch(std::true_type{});
+      else dispatch(std::false_type{});
+    } else {
+      dispatch(std::false_type{});
+    }
     __syncthreads();
   }
 }

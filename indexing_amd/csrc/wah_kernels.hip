@@ -283,28 +283,47 @@ __global__ __launch_bounds__(kBlock) void k_plan(const DescHead* __restrict__ de
       const Column cd = desc_cols(desc)[c];
       uint32_t* tr = tile_runs + cd.sort_tile0;
       const uint32_t tiles = sort_tiles_of(n, sort_tile);
-      // thread t scans a strip of consecutive tiles; loads in batches of 16
+      // thread t scans a strip of consecutive tiles; a strip of up to 64
+      // tiles (2^28 rows in 16384-row tiles) stays in registers between the
+      // sum and the prefix, longer ones are read twice in batches of 16
       const uint32_t per = (tiles + kBlock - 1) / kBlock;
       const uint32_t s0 = t * per < tiles ? t * per : tiles;
       const uint32_t s1 = s0 + per < tiles ? s0 + per : tiles;
-      uint64_t sum = 0;
-      for (uint32_t b = s0; b < s1; b += 16) {
-        uint32_t x[16];
+      constexpr uint32_t kStrip = 64;
+      if (per <= kStrip) {  // block-uniform
+        uint32_t x[kStrip];
+        uint64_t sum = 0;
 #pragma unroll
-        for (int i = 0; i < 16; ++i) x[i] = b + i < s1 ? tr[b + i] : 0u;
+        for (uint32_t i = 0; i < kStrip; ++i) x[i] = s0 + i < s1 ? tr[s0 + i] : 0u;
 #pragma unroll
-        for (int i = 0; i < 16; ++i) sum += x[i];
-      }
-      uint64_t tot;
-      uint64_t run = block_exclusive<OpAdd>(sum, s_scan, &tot);
-      for (uint32_t b = s0; b < s1; b += 16) {
-        uint32_t x[16];
+        for (uint32_t i = 0; i < kStrip; ++i) sum += x[i];
+        uint64_t tot;
+        uint64_t run = block_exclusive<OpAdd>(sum, s_scan, &tot);
 #pragma unroll
-        for (int i = 0; i < 16; ++i) x[i] = b + i < s1 ? tr[b + i] : 0u;
-#pragma unroll
-        for (int i = 0; i < 16; ++i) {
-          if (b + i < s1) tr[b + i] = (uint32_t)run;
+        for (uint32_t i = 0; i < kStrip; ++i) {
+          if (s0 + i < s1) tr[s0 + i] = (uint32_t)run;
           run += x[i];
+        }
+      } else {
+        uint64_t sum = 0;
+        for (uint32_t b = s0; b < s1; b += 16) {
+          uint32_t x[16];
+#pragma unroll
+          for (int i = 0; i < 16; ++i) x[i] = b + i < s1 ? tr[b + i] : 0u;
+#pragma unroll
+          for (int i = 0; i < 16; ++i) sum += x[i];
+        }
+        uint64_t tot;
+        uint64_t run = block_exclusive<OpAdd>(sum, s_scan, &tot);
+        for (uint32_t b = s0; b < s1; b += 16) {
+          uint32_t x[16];
+#pragma unroll
+          for (int i = 0; i < 16; ++i) x[i] = b + i < s1 ? tr[b + i] : 0u;
+#pragma unroll
+          for (int i = 0; i < 16; ++i) {
+            if (b + i < s1) tr[b + i] = (uint32_t)run;
+            run += x[i];
+          }
         }
       }
     }

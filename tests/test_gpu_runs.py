@@ -112,8 +112,13 @@ def key_width_columns():
     runs17 = runs16.copy()
     runs17[100] = 1 << 16
     byte_keys = rng.integers(0, 256, 50000).astype(np.uint32)  # one active pass
+    # 2-byte keys whose low byte is constant: the only active pass sorts on
+    # bits 8..15 (the first pass packs key and row into one staged word)
+    high_byte = (rng.integers(0, 256, 40001) << 8).astype(np.uint32)
+    low_const = ((rng.integers(0, 256, 3 * 16384 + 5) << 8) | 0x37).astype(np.uint32)
     return [("max_65535", top16), ("max_65536", over), ("runs_max_65535", runs16),
-            ("runs_max_65536", runs17), ("one_pass", byte_keys)]
+            ("runs_max_65536", runs17), ("one_pass", byte_keys), ("high_byte_only", high_byte),
+            ("low_byte_constant", low_const)]
 
 
 @pytest.mark.parametrize("name,values", key_width_columns(), ids=[c[0] for c in key_width_columns()])
@@ -121,6 +126,8 @@ def test_key_width_boundary(ctx, name, values):
     print(f"[width] {name} n={values.size} max={int(values.max())}", flush=True)
     got = ctx.build(values, rid_base=3)
     assert got.bound == int(values.max())
+    if name in ("high_byte_only", "low_byte_constant", "one_pass"):
+        assert got.passes == 1  # the constant digit's pass is skipped
     assert_same(got, oracle.build(values, rid_base=3), name)
 
 

@@ -928,7 +928,10 @@ __global__ __launch_bounds__(THREADS, tail_waves_per_eu(ITEMS)) void k_tail(
   __shared__ uint32_t s_tile;
   const Dims dims = desc->dims;
   const Column* cols = desc_cols(desc);
-  const int t = threadIdx.x, w = t / kWave, l = t % kWave;
+  // the wave index in a scalar register: per-wave LDS addresses stay out of
+  // the 64 vector registers
+  const int t = threadIdx.x, l = t % kWave;
+  const int w = __builtin_amdgcn_readfirstlane(t / kWave);
   uint32_t* fault = &hdr->fault;
   if (hdr->mode_cols[RUN ? 1 : 0] == 0) return;  // no column in this mode
 

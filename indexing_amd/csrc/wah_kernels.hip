@@ -388,7 +388,8 @@ __global__ __launch_bounds__(kBlock, 8) void k_runcut(
   __shared__ uint32_t s_wlead[kWaves + 1];  // rows before the first stop of wave w; [kWaves]: past the step
   const Dims dims = desc->dims;
   const Column* cols = desc_cols(desc);
-  const int t = threadIdx.x, w = t / kWave, l = t % kWave;
+  const int t = threadIdx.x, l = t % kWave;
+  const int w = __builtin_amdgcn_readfirstlane(t / kWave);  // scalar: per-wave LDS addresses
   if (hdr->mode_cols[1] == 0) return;  // no run-mode column
   for (uint32_t tile = blockIdx.x; tile < dims.sort_tiles; tile += gridDim.x) {
     const uint32_t c = find_segment(&cols[0].sort_tile0, sizeof(Column) / 4, dims.ncols, tile);
@@ -615,7 +616,8 @@ __global__ __launch_bounds__(THREADS, WAH_SORT_WAVES_PER_EU) void k_onesweep(
   uint32_t* fault = &hdr->fault;
   const Dims dims = desc->dims;
   const Column* cols = desc_cols(desc);
-  const int t = threadIdx.x, w = t / kWave, l = t % kWave;
+  const int t = threadIdx.x, l = t % kWave;
+  const int w = __builtin_amdgcn_readfirstlane(t / kWave);  // scalar: per-wave LDS addresses
   if (hdr->max_active[RUN ? 1 : 0] <= a) return;  // no column has an a-th pass
   uint32_t* counter = hdr->tiles[(RUN ? kCounterRun : 0) + a];
 

@@ -19,6 +19,13 @@ constexpr uint32_t kFlagPrefix = 2u;      // inclusive prefix published
 __device__ __forceinline__ uint32_t lane_id() {
   return __builtin_amdgcn_mbcnt_hi(~0u, __builtin_amdgcn_mbcnt_lo(~0u, 0u));
 }
+// The same value, opaque to the optimiser: an address derived from it is
+// recomputed where it is used (one or two VALU) instead of being hoisted out
+// of a loop and kept, or spilled, across it.  Emits no instruction.
+__device__ __forceinline__ uint32_t opaque(uint32_t x) {
+  asm volatile("" : "+v"(x));
+  return x;
+}
 __device__ __forceinline__ uint64_t ballot(bool p) {
   return __builtin_amdgcn_ballot_w64(p);
 }

@@ -1107,7 +1107,8 @@ __global__ __launch_bounds__(THREADS, tail_waves_per_eu(ITEMS)) void k_tail(
     uint32_t woff, tot;
     {
       // wave totals: one LDS read per lane and a wave scan
-      const uint32_t x = l < WAVES ? s_wtot[l] : 0u;
+      const uint32_t lo = opaque((uint32_t)l);
+      const uint32_t x = lo < WAVES ? s_wtot[lo] : 0u;
       const uint32_t xi = wave_incl_add(x);
       woff = readlane(xi - x, __builtin_amdgcn_readfirstlane(w));
       tot = readlane(xi, WAVES - 1);
@@ -1117,7 +1118,8 @@ __global__ __launch_bounds__(THREADS, tail_waves_per_eu(ITEMS)) void k_tail(
       const uint64_t tile_agg = ((uint64_t)(tot >> 16) << 31) | (tot & 0xFFFFu);
       const uint64_t pre = lookback_wave_sum<WAH_TAIL_LOOKBACK_SLOTS>(status, tile, first_tile,
                                                                       tile_agg, fault);
-      const uint32_t e = readlane(wave_incl_add(l < WAVES ? s_went[l] : 0u), kWave - 1);
+      const uint32_t lo = opaque((uint32_t)l);
+      const uint32_t e = readlane(wave_incl_add(lo < WAVES ? s_went[lo] : 0u), kWave - 1);
       if (l == 0) {
         s_prefix = pre;
         tile_entries[tile] = e;
@@ -1186,7 +1188,7 @@ __global__ __launch_bounds__(THREADS, tail_waves_per_eu(ITEMS)) void k_tail(
       if (bad) atomicOr(fault, 2u);
     }
     const uint32_t Wt = tot & 0xFFFFu;
-    for (uint32_t j = t; j < Wt; j += THREADS) {
+    for (uint32_t j = opaque((uint32_t)t); j < Wt; j += THREADS) {
       const uint32_t o = Wpre + j;
       if (o < 2 * n) out_words[o] = s_words[j + (j >> 4)];
       else atomicOr(fault, 2u);

@@ -1188,10 +1188,31 @@ __global__ __launch_bounds__(THREADS, tail_waves_per_eu(ITEMS)) void k_tail(
       if (bad) atomicOr(fault, 2u);
     }
     const uint32_t Wt = tot & 0xFFFFu;
-    for (uint32_t j = opaque((uint32_t)t); j < Wt; j += THREADS) {
-      const uint32_t o = Wpre + j;
-      if (o < 2 * n) out_words[o] = s_words[j + (j >> 4)];
-      else atomicOr(fault, 2u);
+    if ((uint64_t)Wpre + Wt <= 2ull * n) {
+      // single words up to the first 16-byte boundary, then 4 words per
+      // 16-byte store, then the rest
+      uint32_t* ow = out_words + Wpre;
+      const uint32_t head0 = (4u - (uint32_t)(((uintptr_t)ow >> 2) & 3u)) & 3u;
+      const uint32_t head = head0 < Wt ? head0 : Wt;
+      const uint32_t quads = (Wt - head) / 4u;
+      const uint32_t tt = opaque((uint32_t)t);
+      if (tt < head) ow[tt] = s_words[tt + (tt >> 4)];
+      for (uint32_t qd = tt; qd < quads; qd += THREADS) {
+        const uint32_t j = head + 4u * qd;
+        uint4 v;
+        v.x = s_words[j + (j >> 4)];
+        v.y = s_words[j + 1u + ((j + 1u) >> 4)];
+        v.z = s_words[j + 2u + ((j + 2u) >> 4)];
+        v.w = s_words[j + 3u + ((j + 3u) >> 4)];
+        *reinterpret_cast<uint4*>(ow + j) = v;  // 16-byte aligned: j - head is a multiple of 4
+      }
+      for (uint32_t j = head + 4u * quads + tt; j < Wt; j += THREADS) ow[j] = s_words[j + (j >> 4)];
+    } else {
+      for (uint32_t j = opaque((uint32_t)t); j < Wt; j += THREADS) {
+        const uint32_t o = Wpre + j;
+        if (o < 2 * n) out_words[o] = s_words[j + (j >> 4)];
+        else atomicOr(fault, 2u);
+      }
     }
     if (base + TILE >= n && t == 0) {  // the column's last tile
       counts[cd.index].length = (uint64_t)Wpre + Wt;

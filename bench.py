@@ -21,9 +21,14 @@ Beside it:
                 and lengths 12K); moved_bytes = (2kb + 8)nP + 8n + 4L + 12K
                 with the tail reading (kb + 4)n (run mode: the k run items
                 through the cut, the passes and the tail)
+  parity        the index of the last timed step, digested, against the
+                oracle's digest for the same column (tests/golden/fullsize.json)
+  kernels       per kernel group: mean ms, algorithmic and moved bytes, GB/s
   cpu_baseline  the oracle (a C restatement of the reference's serial `cpu`
-                program, one thread) on a 2^26-value sample of the same
-                distribution, timed on this host
+                program) on the whole 2^28 column, run in a child process
+                pinned to one core while the GPU legs run, with per-stage
+                times, the CPU model and core count, and the literal bubble
+                sort checked against the stable sort on a 2^16 sample
   batched       the mixed 1024-column workload (lengths log-uniform in
                 [2^12, 2^24], cardinality log-uniform in [2, 2^16], half
                 uniform half Zipf(1.0), seed 7), resident in HBM: one batched
@@ -48,7 +53,6 @@ sys.path.insert(0, ROOT)
 METRIC = "indexed Gvalues/s + HBM-roofline % (2^28 u32); p50/p99 serve latency"
 HBM_PEAK_GBS = 8000.0
 N_VALUES = 1 << 28
-CPU_SAMPLE = 1 << 26
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "traffic_onesweep_uniform_2p28.txt")
 
 
@@ -97,20 +101,78 @@ def progress(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-def cpu_baseline(sample_n):
-    import numpy as np
+def cpu_info():
+    model = "unknown"
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    return model, len(os.sched_getaffinity(0)), os.cpu_count()
 
+
+def cpu_baseline_worker(n, core):
+    """Runs in a child process pinned to one core: the oracle (a C
+    restatement of the reference's serial `cpu` program, src/cpu.cpp:176-272)
+    on the whole 2^28 column with the stable sort, per-stage times; then the
+    literal O(n^2) bubble sort of the reference (src/cpu.cpp:180-197) against
+    the stable sort on the first 2^16 values of the 2^20 uniform seed-42
+    column, compared by index digest.  Prints one JSON object."""
+    os.sched_setaffinity(0, {core})
     import indexing_amd as ia
+    from indexing_amd.digest import index_digest
     from oracle import oracle
-    values = ia.generate(sample_n, "uniform", seed=1, lo=0, hi=65535)
+    values = ia.generate(n, "uniform", seed=1, lo=0, hi=65535)
     t0 = time.perf_counter()
-    r = oracle.build(values)
+    r = oracle.build(values, timed=True)
     dt = time.perf_counter() - t0
-    assert r["keycnt"] == np.unique(values).size
-    return {"value": sample_n / dt / 1e9, "unit": "Gvalues/s", "cores": 1, "kind": "port",
-            "sample": f"first 2^{sample_n.bit_length() - 1} values of the same column "
-                      f"(uniform [0,65535], seed 1), oracle build timed single-threaded, "
-                      f"{dt:.2f} s"}
+    full = {"n": n, "seconds": round(dt, 3), "gvalues_s": n / dt / 1e9,
+            "stage_s": {k: round(v, 4) for k, v in r["stage_s"].items()},
+            "k": r["k"], "length": r["length"], "keycnt": r["keycnt"],
+            "index": index_digest(r["keys"], r["offsets"], r["lengths"], r["words"])}
+    del r, values
+    sub = ia.generate(1 << 20, "uniform", seed=42, lo=0, hi=65535)[:1 << 16]
+    t0 = time.perf_counter()
+    b = oracle.build(sub, sort="bubble")
+    tb = time.perf_counter() - t0
+    s = oracle.build(sub, sort="packed")  # (value, rid) words sorted: the stable order
+    dig = [index_digest(x["keys"], x["offsets"], x["lengths"], x["words"]) for x in (b, s)]
+    bubble = {"sample": "first 2^16 values of the 2^20 uniform [0,65535] seed-42 column",
+              "bubble_seconds": round(tb, 3), "bubble_digest": dig[0], "stable_digest": dig[1],
+              "equal": dig[0] == dig[1]}
+    print(json.dumps({"full": full, "bubble_vs_stable": bubble}))
+
+
+def start_cpu_baseline(n):
+    """Starts the CPU baseline in a child process on the last core this
+    process may use (the GPU legs run meanwhile on the others)."""
+    import subprocess
+    core = max(os.sched_getaffinity(0))
+    return core, subprocess.Popen([sys.executable, os.path.abspath(__file__),
+                                   "--cpu-baseline-worker", str(n), str(core)],
+                                  stdout=subprocess.PIPE, text=True)
+
+
+def finish_cpu_baseline(proc, core, gold):
+    out, _ = proc.communicate()
+    if proc.returncode != 0:
+        raise RuntimeError("cpu baseline worker failed")
+    r = json.loads(out.strip().splitlines()[-1])
+    full = r["full"]
+    model, allowed, total = cpu_info()
+    res = {"value": round(full["gvalues_s"], 5), "unit": "Gvalues/s", "cores": 1,
+           "kind": "port",
+           "sample": f"the whole 2^{full['n'].bit_length() - 1}-value column (uniform [0,65535], "
+                     f"seed 1), oracle build with the stable sort on one pinned core "
+                     f"(core {core}), {full['seconds']} s, text parsing excluded",
+           "cpu_model": model, "nproc": allowed, "host_cpus": total,
+           "stage_s": full["stage_s"],
+           "bubble_vs_stable": r["bubble_vs_stable"]}
+    if gold is not None:
+        res["matches_gpu_digest"] = full["index"] == gold
+    return res
 
 
 def bench_batched(ncols, seed=7, reps=3):
@@ -246,6 +308,51 @@ def bench_serve(nreq, seed=11, arena_gib=64):
     return res
 
 
+GOLD_FILE = os.path.join(ROOT, "tests", "golden", "fullsize.json")
+
+
+def golden_record(column, n):
+    """The oracle's counts and index digest for this configuration, when
+    tests/golden/fullsize.json holds it (c1, c2, c3), else None."""
+    try:
+        gold = json.load(open(GOLD_FILE))
+    except OSError:
+        return None, None
+    for name in ("c1", "c2", "c3"):
+        p = gold.get(name, {}).get("params", {})
+        dist = "uniform" if column == "uniform" else "zipf-runs"
+        if p.get("n") == n and p.get("dist") == dist and p.get("seed") == 1:
+            return name, gold[name]
+    return None, None
+
+
+def kernel_table(group_ms, calls, bm, n, k, L, K, mode):
+    """Per kernel group: mean ms, algorithmic bytes (the canonical byte model:
+    histogram 4n, passes 12n / 16n, tail 8n + 4L + 8K, lengths 4K, which sum
+    to B_alg) and the bytes this build moves, each as GB/s."""
+    kb = bm["kb"]
+    names = ["histogram", "pass0", "pass1", "pass2", "pass3", "tail", "lengths"]
+    pass_ms = [group_ms[1 + p] / calls for p in range(4)]
+    P = len(bm["alg_pass"])
+    active = sorted(sorted(range(4), key=lambda p: -pass_ms[p])[:P])
+    alg = {"histogram": 4.0 * n, "tail": 8.0 * n + 4.0 * L + 8.0 * K, "lengths": 4.0 * K}
+    moved = {"histogram": 4.0 * n if mode == 0 else 8.0 * n + 12.0 * k,
+             "tail": ((kb + 4.0) * n if mode == 0 else (kb + 8.0) * k) + 4.0 * L + 8.0 * K,
+             "lengths": 4.0 * K}
+    for i, p in enumerate(active):
+        alg[f"pass{p}"] = bm["alg_pass"][i]
+        moved[f"pass{p}"] = bm["moved_pass"][i]
+    out = {}
+    for name, g in zip(names, group_ms):
+        ms = g / calls
+        if name.startswith("pass") and name not in alg:
+            continue  # an inactive pass (its digit is constant): returns at once
+        a, m = alg.get(name, 0.0), moved.get(name, 0.0)
+        out[name] = {"ms": round(ms, 4), "alg_bytes": a, "alg_GBs": round(a / ms / 1e6, 1),
+                     "moved_bytes": m, "moved_GBs": round(m / ms / 1e6, 1)}
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -262,16 +369,29 @@ def main():
                     help="columns of the batched workload (0 skips it)")
     ap.add_argument("--serve-requests", type=int, default=400,
                     help="requests of the serving workload (0 skips it)")
+    ap.add_argument("--cpu-baseline-worker", nargs=2, type=int, metavar=("N", "CORE"),
+                    help=argparse.SUPPRESS)
     args = ap.parse_args()
+    if args.cpu_baseline_worker:
+        cpu_baseline_worker(*args.cpu_baseline_worker)
+        return
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
               "(the multi-GPU sharding is a design note only)", file=sys.stderr)
         sys.exit(2)
 
+    # the CPU baseline runs meanwhile in a child pinned to one core (the
+    # reference's serial program on the same 2^28 column)
+    cpu = None
+    if not args.no_cpu_baseline and args.column == "uniform":
+        cpu = start_cpu_baseline(args.n)
+        progress(f"cpu baseline started on core {cpu[0]}")
+
     import numpy as np
     import torch
 
     import indexing_amd as ia
+    from indexing_amd.digest import index_digest
 
     n = args.n
     if args.column == "uniform":
@@ -282,32 +402,53 @@ def main():
                            run_mean=4096.0)
         column = "Zipf(1.2) over 2^20 values in runs of mean length 4096, seed 1"
     d_values = torch.from_numpy(host.view(np.int32)).to("cuda:0")
+    del host
     ctx = ia.Context(0)
     # tile shape per length bucket
     tuning = ctx.tuning() if args.no_autotune else ctx.autotune()
     out = ctx.build_device(d_values)                   # allocates outputs + arena
     for _ in range(args.warmup):
         ctx.build_device(d_values, out=out)
+    # every timed step writes its own counts, so a fault in any step is seen
+    step_counts = [torch.zeros(out[4].numel(), dtype=torch.int32, device="cuda:0")
+                   for _ in range(args.steps)]
     torch.cuda.synchronize()
 
     ctx.timing_begin(args.steps)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    for _ in range(args.steps):
-        ctx.build_device(d_values, out=out)
+    for i in range(args.steps):
+        ctx.build_device(d_values, out=out[:4] + (step_counts[i],))
     torch.cuda.synchronize()
     elapsed = time.perf_counter() - t0
     group_ms, calls = ctx.timing_end()
-    counts = ctx.read_counts(out[4])
+    all_counts = [ctx.read_counts(c) for c in step_counts]
+    counts = all_counts[-1]
     rank_mode = ctx.rank_mode
-    assert counts.fault == 0, "device reported a fault"
+    assert all(c.fault == 0 for c in all_counts), "device reported a fault in a timed step"
+    assert len({(c.k, c.length, c.keycnt) for c in all_counts}) == 1, "steps disagree"
     assert calls == args.steps
 
     ms_per_step = elapsed * 1e3 / args.steps
     value = n * args.steps / elapsed / 1e9
 
-    # sizes for the byte model
+    # parity of the timed output: the index of the last timed step against the
+    # oracle's digest for this configuration
     L, K, P = int(counts.length), int(counts.keycnt), int(counts.passes)
+    progress("digesting the output")
+    keys, offs, lens, words = (x.cpu().numpy().view(np.uint32) for x in out[:4])
+    got = index_digest(keys[:K], offs[:K], lens[:K], words[:L])
+    del keys, offs, lens, words
+    gname, grec = golden_record(args.column, n)
+    parity = {"index_digest": got}
+    if grec is not None:
+        parity.update({"reference": f"oracle digest, tests/golden/fullsize.json {gname}",
+                       "equal": got == grec["index"] and
+                       (int(counts.k), L, K) == (grec["k"], grec["length"], grec["keycnt"])})
+    else:
+        parity.update({"reference": "none recorded for this size", "equal": None})
+
+    # sizes for the byte model
     pass_ms = [group_ms[1 + p] / calls for p in range(4)]
     # the P active passes are the P slowest (inactive ones exit at once)
     active = sorted(sorted(range(4), key=lambda p: -pass_ms[p])[:P])
@@ -335,6 +476,7 @@ def main():
         "vs_baseline": None,
         "dtype": "u32",
         "data": "synthetic (seeded splitmix64 column, generated on the host)",
+        "parity": parity,
         "config": {"workload": f"single-column WAH build, n=2^{n.bit_length() - 1} uint32 "
                                f"{column}, resident in HBM",
                    "n": n, "k": int(counts.k), "L": L, "K": K, "radix_passes": P,
@@ -357,20 +499,23 @@ def main():
                            "moved_bytes": b_moved,
                            "moved_frac": round(b_moved / (build_ms_events * 1e-3) / 1e9
                                                / HBM_PEAK_GBS, 4)},
-        "kernel_ms": {name: round(g / calls, 4) for name, g in zip(
-            ["histogram", "pass0", "pass1", "pass2", "pass3", "tail", "lengths"], group_ms)},
+        "kernels": kernel_table(group_ms, calls, bm, n, int(counts.k), L, K, int(counts.mode)),
     }
     ctx.close()
-    del d_values, out
+    del d_values, out, step_counts
     torch.cuda.empty_cache()
     if args.batch_cols:
         result["batched"] = bench_batched(args.batch_cols)
     if args.serve_requests:
         result["serve"] = bench_serve(args.serve_requests)
-    if not args.no_cpu_baseline:
-        progress("cpu baseline")
-        result["cpu_baseline"] = cpu_baseline(min(CPU_SAMPLE, n))
+    if cpu is not None:
+        progress("waiting for the cpu baseline")
+        result["cpu_baseline"] = finish_cpu_baseline(cpu[1], cpu[0],
+                                                     grec["index"] if grec else None)
     print(json.dumps(result))
+    if parity["equal"] is False:
+        print("bench.py: the timed output differs from the oracle's", file=sys.stderr)
+        sys.exit(1)
 
 
 if __name__ == "__main__":

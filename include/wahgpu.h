@@ -86,7 +86,10 @@ const char* wahgpu_last_error(const wahgpu_ctx* ctx);
 void* wahgpu_stream(wahgpu_ctx* ctx);               /* the hipStream_t */
 /* Orders the context's work on a caller's hipStream_t (e.g. the framework's
  * current stream); NULL returns to the context's own stream, a blocking
- * stream that is ordered with the legacy default stream (torch's default). */
+ * stream that is ordered with the legacy default stream (torch's default).
+ * A switch makes the new stream wait (hipStreamWaitEvent) for the work
+ * already queued on the old one, since both use the context's scratch; it is
+ * refused (WAHGPU_EINVAL) while either stream is capturing a graph. */
 int wahgpu_set_stream(wahgpu_ctx* ctx, void* stream);
 int wahgpu_synchronize(wahgpu_ctx* ctx);
 uint64_t wahgpu_arena_bytes(const wahgpu_ctx* ctx);

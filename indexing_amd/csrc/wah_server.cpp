@@ -74,7 +74,7 @@ struct wahgpu_server {
   std::condition_variable work_cv;  // queue non-empty or stop
   std::condition_variable done_cv;  // a request completed
   std::deque<Request*> queue;
-  std::unordered_map<uint64_t, std::unique_ptr<Request>> reqs;
+  std::unordered_map<uint64_t, std::shared_ptr<Request>> reqs;
   uint64_t next_id = 1;
   uint64_t pending = 0;
   bool stop = false;
@@ -289,7 +289,7 @@ int wahgpu_server_submit(wahgpu_server* s, const uint32_t* values, uint64_t n, u
   if (!s || !result || !id || (n && !values)) return WAHGPU_EINVAL;
   if (n > s->slot_values || n > WAHGPU_MAX_VALUES) return WAHGPU_ERANGE;
   if ((uint64_t)rid_base + n > ((uint64_t)1 << 32)) return WAHGPU_ERANGE;
-  auto r = std::make_unique<Request>();
+  auto r = std::make_shared<Request>();
   r->values = values;
   r->n = n;
   r->rid_base = rid_base;
@@ -327,10 +327,14 @@ int wahgpu_server_wait(wahgpu_server* s, uint64_t id) {
   std::unique_lock<std::mutex> lk(s->mu);
   auto it = s->reqs.find(id);
   if (it == s->reqs.end()) return WAHGPU_EINVAL;
-  Request* r = it->second.get();
+  // Another caller's poll or wait on the same id may complete and forget the
+  // request while this one sleeps: the shared_ptr keeps it alive, and only
+  // the entry still in the table is erased.
+  std::shared_ptr<Request> r = it->second;
   s->done_cv.wait(lk, [&] { return r->status != WAHGPU_EBUSY; });
   const int st = r->status;
-  s->reqs.erase(id);
+  auto again = s->reqs.find(id);
+  if (again != s->reqs.end() && again->second == r) s->reqs.erase(again);
   return st;
 }
 

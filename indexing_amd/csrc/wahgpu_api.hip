@@ -23,6 +23,7 @@ struct wahgpu_ctx {
   int device = 0;
   hipStream_t stream = nullptr;      // stream all work is ordered on
   hipStream_t own_stream = nullptr;  // created with the context
+  hipEvent_t switch_ev = nullptr;    // orders a new stream after the old one
   char* arena = nullptr;
   uint64_t arena_cap = 0;
   bool fixed = false;
@@ -389,6 +390,7 @@ void wahgpu_destroy(wahgpu_ctx* c) {
   if (c->stream) (void)hipStreamSynchronize(c->stream);
   if (c->gexec) (void)hipGraphExecDestroy(c->gexec);
   if (c->desc_copied) (void)hipEventDestroy(c->desc_copied);
+  if (c->switch_ev) (void)hipEventDestroy(c->switch_ev);
   if (c->desc_stage) (void)hipHostFree(c->desc_stage);
   if (c->arena) (void)hipFree(c->arena);
   if (c->pinned_counts) (void)hipHostFree(c->pinned_counts);
@@ -399,9 +401,26 @@ void wahgpu_destroy(wahgpu_ctx* c) {
 const char* wahgpu_last_error(const wahgpu_ctx* c) { return c ? c->err.c_str() : ""; }
 void* wahgpu_stream(wahgpu_ctx* c) { return c ? (void*)c->stream : nullptr; }
 
+// All of a context's work shares its arena (scratch, look-back state, the
+// descriptor), so work on the new stream must not start before the work
+// already queued on the old one: an event recorded on the old stream is
+// waited for by the new one.  Arena regrowth and graph teardown then only
+// need to synchronize the current stream.  While either stream is being
+// captured into a graph the event cannot be used, and the switch is refused.
 int wahgpu_set_stream(wahgpu_ctx* c, void* stream) {
   if (!c) return WAHGPU_EINVAL;
-  c->stream = stream ? (hipStream_t)stream : c->own_stream;
+  hipStream_t next = stream ? (hipStream_t)stream : c->own_stream;
+  if (next == c->stream) return WAHGPU_OK;
+  HIPCHK(c, hipSetDevice(c->device));
+  hipStreamCaptureStatus cs_old = hipStreamCaptureStatusNone, cs_new = hipStreamCaptureStatusNone;
+  HIPCHK(c, hipStreamIsCapturing(c->stream, &cs_old));
+  HIPCHK(c, hipStreamIsCapturing(next, &cs_new));
+  if (cs_old != hipStreamCaptureStatusNone || cs_new != hipStreamCaptureStatusNone)
+    return fail(c, WAHGPU_EINVAL, "cannot switch streams while either stream is capturing");
+  if (!c->switch_ev) HIPCHK(c, hipEventCreateWithFlags(&c->switch_ev, hipEventDisableTiming));
+  HIPCHK(c, hipEventRecord(c->switch_ev, c->stream));
+  HIPCHK(c, hipStreamWaitEvent(next, c->switch_ev, 0));
+  c->stream = next;
   return WAHGPU_OK;
 }
 uint64_t wahgpu_arena_bytes(const wahgpu_ctx* c) { return c ? c->arena_cap : 0; }
@@ -926,6 +945,7 @@ int wahgpu_autotune(wahgpu_ctx* c, wahgpu_tuning* out) {
     const uint64_t values = std::max<uint64_t>(len, 1ull << 24);  // per bucket
     const uint32_t ncols = (uint32_t)std::min<uint64_t>(maxcols, values / step);
     std::vector<wahgpu_column> cols(ncols);
+    std::vector<wahgpu_counts> hc(ncols);
     for (uint32_t i = 0; i < ncols; ++i) cols[i] = wahgpu_column{i * step, len, 0u, 0u};
     wahgpu_gen_params gp{};
     gp.dist = WAHGPU_DIST_UNIFORM;
@@ -947,6 +967,17 @@ int wahgpu_autotune(wahgpu_ctx* c, wahgpu_tuning* out) {
         if (rc == WAHGPU_OK && hipEventSynchronize(ev[1]) != hipSuccess) rc = WAHGPU_EHIP;
         float ms = 0.f;
         if (rc == WAHGPU_OK) (void)hipEventElapsedTime(&ms, ev[0], ev[1]);
+        // a build that reported a fault is not a valid timing: fail the tune
+        if (rc == WAHGPU_OK &&
+            hipMemcpy(hc.data(), d_c, ncols * sizeof(wahgpu_counts), hipMemcpyDeviceToHost) !=
+                hipSuccess)
+          rc = WAHGPU_EHIP;
+        for (uint32_t i = 0; rc == WAHGPU_OK && i < ncols; ++i)
+          if (hc[i].fault) {
+            cleanup();
+            return fail(c, WAHGPU_EHIP, "autotune: shape " + std::to_string(sh) +
+                                            " reported fault " + std::to_string(hc[i].fault));
+          }
         if (rep > 0 && ms < ms_best) ms_best = ms;  // first run warms up
       }
       c->tune_ms[b][sh] = ms_best;

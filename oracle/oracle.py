@@ -44,6 +44,9 @@ def lib():
             "wah_oracle_segmented_inclusive_scan": (None, [vp, vp, vp, sz]),
             "wah_oracle_build": (ctypes.c_int, [vp, sz, u32, ctypes.c_int, vp, vp, vp, vp,
                                                 ctypes.POINTER(Counts)]),
+            "wah_oracle_build_timed": (ctypes.c_int, [vp, sz, u32, ctypes.c_int, vp, vp, vp,
+                                                      vp, ctypes.POINTER(Counts),
+                                                      ctypes.POINTER(ctypes.c_double)]),
             "wah_oracle_decoded_bitmap": (sz, [vp, sz, ctypes.c_char_p, sz]),
         }
         for name, (res, args) in sig.items():
@@ -61,8 +64,12 @@ def _u32(a):
     return np.ascontiguousarray(a, dtype=np.uint32).copy()
 
 
-def build(values, rid_base=0, sort="packed"):
-    """Whole pipeline of src/cpu.cpp:313-328 -> dict of numpy arrays + counts."""
+STAGES = ("sort", "chunks", "merge", "fills", "fuse", "column_length")
+
+
+def build(values, rid_base=0, sort="packed", timed=False):
+    """Whole pipeline of src/cpu.cpp:313-328 -> dict of numpy arrays + counts
+    (with timed=True also "stage_s": seconds per stage, STAGES order)."""
     v = _u32(values)
     n = v.size
     cap = max(n, 1)
@@ -70,13 +77,17 @@ def build(values, rid_base=0, sort="packed"):
     words = np.zeros(2 * cap, np.uint32)
     c = Counts()
     mode = SORT_BUBBLE if sort == "bubble" else SORT_PACKED
-    rc = lib().wah_oracle_build(_p(v), n, rid_base, mode, _p(keys), _p(offs), _p(lens),
-                                _p(words), ctypes.byref(c))
+    st = (ctypes.c_double * len(STAGES))()
+    rc = lib().wah_oracle_build_timed(_p(v), n, rid_base, mode, _p(keys), _p(offs), _p(lens),
+                                      _p(words), ctypes.byref(c), st)
     if rc != 0:
         raise MemoryError("oracle build failed")
     K, L = int(c.keycnt), int(c.length)
-    return {"keys": keys[:K], "offsets": offs[:K], "lengths": lens[:K], "words": words[:L],
-            "k": int(c.k), "length": L, "keycnt": K}
+    out = {"keys": keys[:K], "offsets": offs[:K], "lengths": lens[:K], "words": words[:L],
+           "k": int(c.k), "length": L, "keycnt": K}
+    if timed:
+        out["stage_s"] = dict(zip(STAGES, list(st)))
+    return out
 
 
 def sort_rids_by_value(values, rid_base=0, sort="packed"):

@@ -18,6 +18,7 @@
 
 #include <stdlib.h>
 #include <string.h>
+#include <time.h>
 
 /* ---- sort_rids_by_value ------------------------------------------------- */
 
@@ -196,11 +197,21 @@ void wah_oracle_segmented_inclusive_scan(const uint32_t* in, const uint32_t* hea
   }
 }
 
+static double now_s(void) {
+  struct timespec ts;
+  clock_gettime(CLOCK_MONOTONIC, &ts);
+  return (double)ts.tv_sec + 1e-9 * (double)ts.tv_nsec;
+}
+
 /* ---- whole build: src/cpu.cpp:313-328 ----------------------------------- */
-int wah_oracle_build(const uint32_t* values, size_t n, uint32_t rid_base,
-                     int sort_mode, uint32_t* keys, uint32_t* offsets,
-                     uint32_t* lengths, uint32_t* words, wah_oracle_counts* out) {
+int wah_oracle_build_timed(const uint32_t* values, size_t n, uint32_t rid_base,
+                           int sort_mode, uint32_t* keys, uint32_t* offsets,
+                           uint32_t* lengths, uint32_t* words, wah_oracle_counts* out,
+                           double* stage_s) {
+  double t[7] = {0};
   out->k = out->length = out->keycnt = 0;
+  if (stage_s)
+    for (int i = 0; i < WAH_ORACLE_STAGES; ++i) stage_s[i] = 0.0;
   if (n == 0) return 0;
   uint32_t* input = (uint32_t*)malloc(n * sizeof(uint32_t));
   uint32_t* rids = (uint32_t*)malloc(n * sizeof(uint32_t));
@@ -211,19 +222,35 @@ int wah_oracle_build(const uint32_t* values, size_t n, uint32_t rid_base,
     return -1;
   }
   memcpy(input, values, n * sizeof(uint32_t));                   /* :313 */
+  t[0] = now_s();
   wah_oracle_sort_rids_by_value(input, rids, n, rid_base, sort_mode); /* :318 */
+  t[1] = now_s();
   wah_oracle_produce_chunk_id_literals(rids, chids, lits, n);    /* :319 */
+  t[2] = now_s();
   size_t k = wah_oracle_merge_lit_by_val_chids(input, chids, lits, n); /* :320 */
+  t[3] = now_s();
   wah_oracle_produce_fills(input, chids, k);                     /* :322 */
+  t[4] = now_s();
   size_t len = wah_oracle_fuse_fill_literals(chids, lits, words, k); /* :323-324 */
+  t[5] = now_s();
   size_t keycnt = wah_oracle_compute_column_length(input, chids, offsets,
                                                    lengths, k);  /* :325-326 */
   memcpy(keys, input, keycnt * sizeof(uint32_t));                /* keys = input[i], :332 */
+  t[6] = now_s();
+  if (stage_s)
+    for (int i = 0; i < WAH_ORACLE_STAGES; ++i) stage_s[i] = t[i + 1] - t[i];
   out->k = k;
   out->length = len;
   out->keycnt = keycnt;
   free(input); free(rids); free(chids); free(lits);
   return 0;
+}
+
+int wah_oracle_build(const uint32_t* values, size_t n, uint32_t rid_base,
+                     int sort_mode, uint32_t* keys, uint32_t* offsets,
+                     uint32_t* lengths, uint32_t* words, wah_oracle_counts* out) {
+  return wah_oracle_build_timed(values, n, rid_base, sort_mode, keys, offsets, lengths,
+                                words, out, NULL);
 }
 
 /* ---- decoded_bitmap: src/cpu.cpp:47-74 ---------------------------------- */

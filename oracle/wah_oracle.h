@@ -92,6 +92,16 @@ int wah_oracle_build(const uint32_t* values, size_t n, uint32_t rid_base,
                      int sort_mode, uint32_t* keys, uint32_t* offsets,
                      uint32_t* lengths, uint32_t* words, wah_oracle_counts* out);
 
+/* The same, with the seconds of each stage (CLOCK_MONOTONIC) written to
+ * stage_s[0..5]: sort, chunk ids + literals, merge, fills, fuse, column
+ * length (the buckets the reference's `phases` prints, src/phases.cpp:714-1127).
+ * stage_s may be NULL. */
+#define WAH_ORACLE_STAGES 6
+int wah_oracle_build_timed(const uint32_t* values, size_t n, uint32_t rid_base,
+                           int sort_mode, uint32_t* keys, uint32_t* offsets,
+                           uint32_t* lengths, uint32_t* words, wah_oracle_counts* out,
+                           double* stage_s);
+
 /* src/cpu.cpp:47-74: '0'/'1' text of the bitmap, trailing '0's trimmed.
  * Writes at most cap bytes (no terminator) and returns the full length. */
 size_t wah_oracle_decoded_bitmap(const uint32_t* words, size_t len, char* out,

@@ -107,12 +107,31 @@ def test_build_device_matches_host_build(ctx):
     assert np.array_equal(u32(words)[:L], ref["words"])
 
 
-def test_repeat_is_deterministic(ctx):
-    values = ia.generate(1 << 20, "uniform", seed=42, lo=0, hi=65535)
-    a = ctx.build(values)
-    b = ctx.build(values)
-    for name in ("keys", "offsets", "lengths", "words"):
-        assert np.array_equal(getattr(a, name), getattr(b, name))
+def test_builds_alternating_on_two_streams(ctx):
+    """Builds queued on two torch streams in turn share the context's
+    scratch: each switch orders the new stream after the old one.  Growing
+    sizes make the arena regrow while the other stream's build may still
+    run."""
+    import torch
+    streams = [torch.cuda.Stream(), torch.cuda.Stream()]
+    vals = [ia.generate(n, "uniform", seed=i, lo=0, hi=65535)
+            for i, n in enumerate((100003, 300007, 1000033, 3000017, 200001, 2500009))]
+    refs = [oracle.build(v) for v in vals]
+    outs = []
+    for i, v in enumerate(vals):
+        with torch.cuda.stream(streams[i % 2]):
+            d = dev(v)
+            outs.append((d, ctx.build_device(d)))
+    torch.cuda.synchronize()
+    for i, ((d, (keys, offs, lens, words, counts)), ref) in enumerate(zip(outs, refs)):
+        c = ctx.read_counts(counts)
+        assert c.fault == 0
+        K, L = ref["keycnt"], ref["length"]
+        assert (c.k, c.length, c.keycnt) == (ref["k"], L, K), i
+        assert np.array_equal(u32(keys)[:K], ref["keys"]), i
+        assert np.array_equal(u32(offs)[:K], ref["offsets"]), i
+        assert np.array_equal(u32(lens)[:K], ref["lengths"]), i
+        assert np.array_equal(u32(words)[:L], ref["words"]), i
 
 
 # ---- stages (the WITH_CPU_TESTS comparisons of src/phases.cpp) -------------

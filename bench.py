@@ -24,6 +24,8 @@ Beside it:
   parity        the index of the last timed step, digested, against the
                 oracle's digest for the same column (tests/golden/fullsize.json)
   kernels       per kernel group: mean ms, algorithmic and moved bytes, GB/s
+  ingest        the column as a text value file: write time, one serial parse
+                and the parallel parse (host threads), both checked equal
   cpu_baseline  the oracle (a C restatement of the reference's serial `cpu`
                 program) on the whole 2^28 column, run in a child process
                 pinned to one core while the GPU legs run, with per-stage
@@ -173,6 +175,37 @@ def finish_cpu_baseline(proc, core, gold):
     if gold is not None:
         res["matches_gpu_digest"] = full["index"] == gold
     return res
+
+
+def bench_ingest(values, threads):
+    """Text value file of the column (the generator's format, one value per
+    line) written and parsed back: one serial pass (`ifstream >>`
+    semantics) and the parallel parse, checked equal to the column."""
+    import tempfile
+
+    import numpy as np
+
+    import indexing_amd as ia
+    fd, path = tempfile.mkstemp(suffix=".txt")
+    os.close(fd)
+    try:
+        t0 = time.perf_counter()
+        ia.write_values_text(path, values, threads=threads)
+        tw = time.perf_counter() - t0
+        size = os.path.getsize(path)
+        res = {"file_bytes": size, "write_s": round(tw, 3), "threads": threads}
+        for name, th in (("serial", 1), ("parallel", threads)):
+            t0 = time.perf_counter()
+            got = ia.read_values_text(path, threads=th)
+            dt = time.perf_counter() - t0
+            res[f"{name}_parse_s"] = round(dt, 3)
+            res[f"{name}_GBs"] = round(size / dt / 1e9, 2)
+            res[f"{name}_equal"] = bool(np.array_equal(got, values))
+            del got
+        res["speedup"] = round(res["serial_parse_s"] / res["parallel_parse_s"], 2)
+        return res
+    finally:
+        os.remove(path)
 
 
 def bench_batched(ncols, seed=7, reps=3):
@@ -369,6 +402,8 @@ def main():
                     help="columns of the batched workload (0 skips it)")
     ap.add_argument("--serve-requests", type=int, default=400,
                     help="requests of the serving workload (0 skips it)")
+    ap.add_argument("--no-ingest", dest="ingest", action="store_false",
+                    help="skip the text value-file write / parse timing")
     ap.add_argument("--cpu-baseline-worker", nargs=2, type=int, metavar=("N", "CORE"),
                     help=argparse.SUPPRESS)
     args = ap.parse_args()
@@ -402,6 +437,12 @@ def main():
                            run_mean=4096.0)
         column = "Zipf(1.2) over 2^20 values in runs of mean length 4096, seed 1"
     d_values = torch.from_numpy(host.view(np.int32)).to("cuda:0")
+    ingest = None
+    if args.ingest:
+        progress("text ingest")
+        # leave the CPU baseline's core alone
+        ingest = bench_ingest(host, max(1, min(16, len(os.sched_getaffinity(0)) -
+                                               (1 if cpu else 0))))
     del host
     ctx = ia.Context(0)
     # tile shape per length bucket
@@ -501,6 +542,8 @@ def main():
                                                / HBM_PEAK_GBS, 4)},
         "kernels": kernel_table(group_ms, calls, bm, n, int(counts.k), L, K, int(counts.mode)),
     }
+    if ingest:
+        result["ingest"] = ingest
     ctx.close()
     del d_values, out, step_counts
     torch.cuda.empty_cache()

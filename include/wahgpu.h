@@ -345,11 +345,64 @@ int wahgpu_merger_export(const wahgpu_merger* m, uint32_t* keys, uint32_t* offse
 int wahgpu_merger_key_words(const wahgpu_merger* m, uint32_t key,
                             const uint32_t** words, uint64_t* len);
 
+/* ---- query side (device pointers) ---------------------------------------
+ * The reference decodes a key's words on the host, one '0'/'1' character per
+ * row (decoded_bitmap, src/cpu.cpp:47-74; the merger's print,
+ * src/gpu.cpp:151-158).  These run on the device over the index as the
+ * builds leave it.  A row bitmap holds row r in bit r % 32 of word r / 32,
+ * (nrows + 31) / 32 words; rows past nrows are dropped.  Row ids are the
+ * build's (rid_base included).  Each call synchronizes its stream before it
+ * returns. */
+
+/* One key's WAH words -> its row bitmap (d_bitmap is overwritten).
+ * *last_row (may be NULL) = one past the last row the words set: the length
+ * of the reference's decoded string, whose trailing '0's are trimmed. */
+int wahgpu_decode_device(wahgpu_ctx* ctx, const uint32_t* d_words, uint64_t len,
+                         uint64_t nrows, uint32_t* d_bitmap, uint64_t* last_row);
+
+/* Range query on an equality index (keys ascending; offsets, lengths and
+ * words as a build writes them): d_bitmap = OR of the row bitmaps of every
+ * key in [lo, hi].  *keys_matched (may be NULL) = keys in the range. */
+int wahgpu_query_range_device(wahgpu_ctx* ctx, const uint32_t* d_keys,
+                              const uint32_t* d_offsets, const uint32_t* d_lengths,
+                              uint64_t keycnt, const uint32_t* d_words, uint32_t lo,
+                              uint32_t hi, uint64_t nrows, uint32_t* d_bitmap,
+                              uint64_t* keys_matched);
+
+/* Row bitmap -> WAH words in the build's canonical form (src/cpu.cpp:223-257:
+ * per nonempty chunk a 0-fill of the chunks skipped, when nonzero, then the
+ * literal; no 1-fills, no trailing fill).  *len = words; fails with
+ * WAHGPU_EINVAL when *len > cap (no word past cap is written). */
+int wahgpu_encode_device(wahgpu_ctx* ctx, const uint32_t* d_bitmap, uint64_t nrows,
+                         uint32_t* d_words, uint64_t cap, uint64_t* len);
+
+/* Two WAH streams over rows [0, nrows) -> the canonical WAH words of
+ * a op b.  cap = 2 (la + lb) always suffices. */
+#define WAHGPU_OP_AND 0
+#define WAHGPU_OP_OR 1
+#define WAHGPU_OP_XOR 2
+#define WAHGPU_OP_ANDNOT 3 /* a AND NOT b */
+int wahgpu_combine_device(wahgpu_ctx* ctx, int op, const uint32_t* d_a, uint64_t la,
+                          const uint32_t* d_b, uint64_t lb, uint64_t nrows, uint32_t* d_out,
+                          uint64_t cap, uint64_t* len);
+
 /* ---- host-side formats (no GPU needed) --------------------------------- */
 /* Value files as the reference reads them (src/cpu.cpp:297-303): whitespace-
  * separated decimal, parsed like `ifstream >> uint32_t`, stopping silently at
- * the first failed extraction.  *values is malloc'ed; free with wahgpu_free. */
+ * the first failed extraction.  *values is malloc'ed; free with wahgpu_free.
+ * The file is parsed in parallel (up to 16 threads): it is cut into spans at
+ * white space, each span is parsed like the loop, and the values are those
+ * before the first failed extraction, exactly as a serial read. */
 int wahgpu_read_values_text(const char* path, uint32_t** values, uint64_t* n);
+/* The same with a thread count (1 = one serial pass, 0 = as many as the
+ * host has, at most 16). */
+int wahgpu_read_values_text_threads(const char* path, int threads, uint32_t** values,
+                                    uint64_t* n);
+/* Writes values as the generator does (src/generate_test_data.cpp:48-54):
+ * decimal, separated by sep, then a newline; formatted on `threads` threads
+ * (0 = as many as the host has, at most 16). */
+int wahgpu_write_values_text(const char* path, const uint32_t* values, uint64_t n,
+                             const char* sep, int threads);
 /* Raw little-endian uint32 files (opt-in extension). */
 int wahgpu_read_values_binary(const char* path, uint32_t** values, uint64_t* n);
 void wahgpu_free(void* p);

@@ -22,6 +22,7 @@ from ._native import WahError, MAX_VALUES
 
 __all__ = [
     "Context", "Server", "Index", "pinned_empty", "WahError", "MAX_VALUES", "Merger", "read_values_text",
+    "write_values_text",
     "read_values_binary", "decode_bitmap", "generate", "device_count",
 ]
 
@@ -349,6 +350,51 @@ class Context:
         K = K.value
         return keys[:K], offs[:K], lens[:K]
 
+    # -- query side (device tensors) ------------------------------------------
+    def decode_device(self, words, nrows):
+        """A key's WAH words -> (row bitmap tensor of (nrows + 31) // 32 words,
+        one past its last set row)."""
+        bm = self._empty((int(nrows) + 31) // 32)
+        last = ctypes.c_uint64(0)
+        self._bind_stream()
+        self._check(self._lib.wahgpu_decode_device(self._h, _ptr(words), int(words.numel()),
+                                                   int(nrows), _ptr(bm), ctypes.byref(last)))
+        return bm, int(last.value)
+
+    def query_range_device(self, keys, offsets, lengths, words, lo, hi, nrows):
+        """OR of the row bitmaps of the keys in [lo, hi] of a device index ->
+        (row bitmap tensor, keys matched)."""
+        bm = self._empty((int(nrows) + 31) // 32)
+        m = ctypes.c_uint64(0)
+        self._bind_stream()
+        self._check(self._lib.wahgpu_query_range_device(
+            self._h, _ptr(keys), _ptr(offsets), _ptr(lengths), int(keys.numel()), _ptr(words),
+            int(lo), int(hi), int(nrows), _ptr(bm), ctypes.byref(m)))
+        return bm, int(m.value)
+
+    def encode_device(self, bitmap, nrows, cap=None):
+        """Row bitmap tensor -> canonical WAH words (a device tensor)."""
+        cap = 2 * ((int(nrows) + 30) // 31) if cap is None else int(cap)
+        out = self._empty(max(cap, 1))
+        L = ctypes.c_uint64(0)
+        self._bind_stream()
+        self._check(self._lib.wahgpu_encode_device(self._h, _ptr(bitmap), int(nrows), _ptr(out),
+                                                   cap, ctypes.byref(L)))
+        return out[:L.value]
+
+    def combine_device(self, op, a, b, nrows, cap=None):
+        """op in {"and", "or", "xor", "andnot"} of two WAH word tensors over
+        rows [0, nrows) -> canonical WAH words (a device tensor)."""
+        code = {"and": N.OP_AND, "or": N.OP_OR, "xor": N.OP_XOR, "andnot": N.OP_ANDNOT}[op]
+        cap = 2 * (int(a.numel()) + int(b.numel())) if cap is None else int(cap)
+        out = self._empty(max(cap, 1))
+        L = ctypes.c_uint64(0)
+        self._bind_stream()
+        self._check(self._lib.wahgpu_combine_device(
+            self._h, code, _ptr(a), int(a.numel()), _ptr(b), int(b.numel()), int(nrows),
+            _ptr(out), cap, ctypes.byref(L)))
+        return out[:L.value]
+
     # -- primitives ----------------------------------------------------------
     def stream_compaction(self, data, valid):
         n = data.numel()
@@ -390,9 +436,21 @@ def _read(fn, path):
         L.wahgpu_free(p)
 
 
-def read_values_text(path):
-    """Whitespace-separated decimal uint32s, read like `ifstream >> uint32_t`."""
-    return _read(N.lib().wahgpu_read_values_text, path)
+def read_values_text(path, threads=0):
+    """Whitespace-separated decimal uint32s, read like `ifstream >> uint32_t`
+    (parsed in parallel; threads=1 is one serial pass, 0 = up to 16)."""
+    L = N.lib()
+    return _read(lambda p, v, n: L.wahgpu_read_values_text_threads(p, int(threads), v, n), path)
+
+
+def write_values_text(path, values, sep="\n", threads=0):
+    """Writes values like the `generate` program: decimal, separated by sep,
+    then a newline."""
+    v = _as_u32(values)
+    rc = N.lib().wahgpu_write_values_text(str(path).encode(), v.ctypes.data_as(ctypes.c_void_p),
+                                          v.size, sep.encode(), int(threads))
+    if rc != N.OK:
+        raise WahError(rc, f"cannot write {path}")
 
 
 def read_values_binary(path):

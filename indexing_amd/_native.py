@@ -17,6 +17,7 @@ MAX_VALUES = (1 << 30) - 1
 TIMING_GROUPS = 7
 
 DIST_UNIFORM, DIST_ZIPF, DIST_ZIPF_RUNS = 0, 1, 2
+OP_AND, OP_OR, OP_XOR, OP_ANDNOT = 0, 1, 2, 3
 
 c_u32p = ctypes.POINTER(ctypes.c_uint32)
 c_u64p = ctypes.POINTER(ctypes.c_uint64)
@@ -145,8 +146,22 @@ SIGNATURES = {
     "wahgpu_merger_export": (ctypes.c_int, [vp, vp, vp, vp, vp]),
     "wahgpu_merger_key_words": (ctypes.c_int, [vp, ctypes.c_uint32, ctypes.POINTER(c_u32p),
                                                c_u64p]),
+    "wahgpu_decode_device": (ctypes.c_int, [vp, vp, ctypes.c_uint64, ctypes.c_uint64, vp,
+                                            c_u64p]),
+    "wahgpu_query_range_device": (ctypes.c_int, [vp, vp, vp, vp, ctypes.c_uint64, vp,
+                                                 ctypes.c_uint32, ctypes.c_uint32,
+                                                 ctypes.c_uint64, vp, c_u64p]),
+    "wahgpu_encode_device": (ctypes.c_int, [vp, vp, ctypes.c_uint64, vp, ctypes.c_uint64,
+                                            c_u64p]),
+    "wahgpu_combine_device": (ctypes.c_int, [vp, ctypes.c_int, vp, ctypes.c_uint64, vp,
+                                             ctypes.c_uint64, ctypes.c_uint64, vp,
+                                             ctypes.c_uint64, c_u64p]),
     "wahgpu_read_values_text": (ctypes.c_int, [ctypes.c_char_p, ctypes.POINTER(c_u32p),
                                                c_u64p]),
+    "wahgpu_read_values_text_threads": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_int,
+                                                       ctypes.POINTER(c_u32p), c_u64p]),
+    "wahgpu_write_values_text": (ctypes.c_int, [ctypes.c_char_p, vp, ctypes.c_uint64,
+                                                ctypes.c_char_p, ctypes.c_int]),
     "wahgpu_read_values_binary": (ctypes.c_int, [ctypes.c_char_p, ctypes.POINTER(c_u32p),
                                                  c_u64p]),
     "wahgpu_free": (None, [vp]),

@@ -5,7 +5,13 @@
 #include <stdlib.h>
 #include <string.h>
 
+#include <fcntl.h>
+#include <sys/mman.h>
+#include <sys/stat.h>
+#include <unistd.h>
+
 #include <algorithm>
+#include <thread>
 #include <vector>
 
 #include "wahgpu.h"
@@ -64,6 +70,96 @@ struct Zipf {
   }
 };
 
+// One `ifstream >> uint32_t` extraction loop (src/cpu.cpp:298-302) over
+// [p, e): values go to out (room for (e - p + 1) / 2 of them: a value takes
+// a digit and, but for the last, a separator); *failed is set when an
+// extraction failed before the end (the loop ends there).
+uint64_t parse_span(const unsigned char* p, const unsigned char* e, uint32_t* out, bool* failed) {
+  static const struct SpaceTable {
+    bool t[256];
+    SpaceTable() : t() {
+      for (int c = 0; c < 256; ++c) t[c] = is_space((unsigned char)c);
+    }
+  } space;
+  uint64_t cnt = 0;
+  *failed = false;
+  for (;;) {
+    while (p < e && space.t[*p]) ++p;
+    if (p == e) break;
+    bool neg = false;
+    if (*p == '+' || *p == '-') { neg = *p == '-'; ++p; }
+    if (p == e || (unsigned)(*p - '0') > 9u) { *failed = true; break; }
+    // up to 10 digits cannot overflow 64 bits; longer tokens (leading
+    // zeros) continue with a saturating accumulator
+    const unsigned char* q = p;
+    const unsigned char* stop = e - p > 10 ? p + 10 : e;
+    uint64_t acc = 0;
+    while (q < stop && (unsigned)(*q - '0') <= 9u) acc = acc * 10u + (uint64_t)(*q++ - '0');
+    while (q < e && (unsigned)(*q - '0') <= 9u) {
+      acc = acc * 10u + (uint64_t)(*q++ - '0');
+      if (acc > 0xFFFFFFFFull) acc = 0x100000000ull;
+    }
+    p = q;
+    if (acc > 0xFFFFFFFFull) { *failed = true; break; }
+    out[cnt++] = neg ? (uint32_t)(0u - (uint32_t)acc) : (uint32_t)acc;
+  }
+  return cnt;
+}
+
+// A read-only mapping of a whole file (empty files map to nothing).
+struct Mapped {
+  const unsigned char* p = nullptr;
+  size_t size = 0;
+  int fd = -1;
+  bool open(const char* path) {
+    fd = ::open(path, O_RDONLY);
+    if (fd < 0) return false;
+    struct stat st;
+    if (fstat(fd, &st) != 0) return false;
+    size = (size_t)st.st_size;
+    if (size == 0) return true;
+    // populated at once: one pass in the kernel instead of a fault per page
+    void* m = mmap(nullptr, size, PROT_READ, MAP_PRIVATE | MAP_POPULATE, fd, 0);
+    if (m == MAP_FAILED) return false;
+    p = static_cast<const unsigned char*>(m);
+    return true;
+  }
+  ~Mapped() {
+    if (p) munmap(const_cast<unsigned char*>(p), size);
+    if (fd >= 0) close(fd);
+  }
+};
+
+// Heap memory (released with free()) backed by 2 MiB pages where the kernel
+// allows them: a gigabyte of parsed values then takes hundreds of page
+// faults instead of a quarter million.
+void* alloc_large(size_t bytes) {
+  constexpr size_t kHuge = (size_t)2 << 20;
+  if (bytes < kHuge) return malloc(bytes ? bytes : 1);
+  const size_t cap = (bytes + kHuge - 1) / kHuge * kHuge;
+  void* p = aligned_alloc(kHuge, cap);
+  if (p) (void)madvise(p, cap, MADV_HUGEPAGE);
+  return p;
+}
+
+unsigned auto_threads(int threads) {
+  if (threads > 0) return (unsigned)threads;
+  unsigned h = std::thread::hardware_concurrency();
+  return h ? std::min(h, 16u) : 4u;
+}
+
+// Decimal digits of v, written at out; returns their number.
+inline int format_u32(uint32_t v, char* out) {
+  char tmp[10];
+  int n = 0;
+  do {
+    tmp[n++] = (char)('0' + v % 10u);
+    v /= 10u;
+  } while (v);
+  for (int i = 0; i < n; ++i) out[i] = tmp[n - 1 - i];
+  return n;
+}
+
 }  // namespace
 
 extern "C" {
@@ -73,37 +169,109 @@ extern "C" {
 // white space; optional '+'/'-'; at least one decimal digit; a value above
 // 2^32-1 fails the extraction; '-' negates modulo 2^32.  The first failed
 // extraction ends the input (the `while (source >> next)` loop).
-int wahgpu_read_values_text(const char* path, uint32_t** values, uint64_t* n) {
-  if (!path || !values || !n) return WAHGPU_EINVAL;
+//
+// In parallel: the mapped file is cut into one span per thread at white
+// space, so no token straddles two spans; each thread parses its span as the
+// loop would; the values are those of the spans before the first span whose
+// parse failed, plus that span's values before its failure.
+int wahgpu_read_values_text_threads(const char* path, int threads, uint32_t** values,
+                                    uint64_t* n) {
+  if (!path || !values || !n || threads < 0) return WAHGPU_EINVAL;
   *values = nullptr;
   *n = 0;
-  std::vector<unsigned char> buf;
-  if (!slurp(path, &buf)) return WAHGPU_EIO;
-  size_t cap = buf.size() / 2 + 1;  // every value takes >= 2 bytes but the last
-  uint32_t* out = (uint32_t*)malloc(cap * sizeof(uint32_t));
-  if (!out) return WAHGPU_ENOMEM;
-  const unsigned char* p = buf.data();
-  const unsigned char* e = p + buf.size();
-  uint64_t cnt = 0;
-  for (;;) {
-    while (p < e && is_space(*p)) ++p;
-    if (p == e) break;
-    bool neg = false;
-    if (*p == '+' || *p == '-') { neg = *p == '-'; ++p; }
-    if (p == e || *p < '0' || *p > '9') break;
-    uint64_t acc = 0;
-    bool overflow = false;
-    while (p < e && *p >= '0' && *p <= '9') {
-      acc = acc * 10u + (uint64_t)(*p - '0');
-      if (acc > 0xFFFFFFFFull) { overflow = true; acc = 0xFFFFFFFFull; }
-      ++p;
-    }
-    if (overflow) break;
-    out[cnt++] = neg ? (uint32_t)(0u - (uint32_t)acc) : (uint32_t)acc;
+  Mapped f;
+  if (!f.open(path)) return WAHGPU_EIO;
+  const unsigned char* buf = f.p;
+  const size_t size = f.size;
+  unsigned T = auto_threads(threads);
+  if (threads == 0 && size < ((size_t)1 << 20)) T = 1;  // not worth the threads
+  if ((size_t)T > size) T = size ? (unsigned)size : 1u;
+  std::vector<size_t> cut(T + 1, size);
+  cut[0] = 0;
+  for (unsigned i = 1; i < T; ++i) {
+    size_t b = std::max(cut[i - 1], size / T * i);
+    while (b < size && !is_space(buf[b])) ++b;
+    cut[i] = b;
   }
+  std::vector<uint32_t*> part(T, nullptr);
+  std::vector<uint64_t> cnt(T, 0);
+  std::vector<char> failed(T, 0), oom(T, 0);
+  auto work = [&](unsigned i) {
+    const size_t len = cut[i + 1] - cut[i];
+    part[i] = (uint32_t*)alloc_large(((len + 1) / 2 + 1) * sizeof(uint32_t));
+    if (!part[i]) { oom[i] = 1; return; }
+    bool fl = false;
+    cnt[i] = len ? parse_span(buf + cut[i], buf + cut[i + 1], part[i], &fl) : 0;
+    failed[i] = fl;
+  };
+  std::vector<std::thread> th;
+  for (unsigned i = 1; i < T; ++i) th.emplace_back(work, i);
+  work(0);
+  for (auto& t : th) t.join();
+  auto free_parts = [&]() {
+    for (auto q : part) free(q);
+  };
+  for (unsigned i = 0; i < T; ++i)
+    if (oom[i]) { free_parts(); return WAHGPU_ENOMEM; }
+  unsigned last = T;  // spans [0, last) contribute
+  for (unsigned i = 0; i < T; ++i)
+    if (failed[i]) { last = i + 1; break; }
+  std::vector<uint64_t> at(last + 1, 0);
+  for (unsigned i = 0; i < last; ++i) at[i + 1] = at[i] + cnt[i];
+  uint32_t* out = (uint32_t*)alloc_large((at[last] ? at[last] : 1) * sizeof(uint32_t));
+  if (!out) { free_parts(); return WAHGPU_ENOMEM; }
+  th.clear();
+  for (unsigned i = 1; i < last; ++i)
+    th.emplace_back([&, i]() { memcpy(out + at[i], part[i], cnt[i] * sizeof(uint32_t)); });
+  if (last) memcpy(out, part[0], cnt[0] * sizeof(uint32_t));
+  for (auto& t : th) t.join();
+  free_parts();
   *values = out;
-  *n = cnt;
+  *n = at[last];
   return WAHGPU_OK;
+}
+
+int wahgpu_read_values_text(const char* path, uint32_t** values, uint64_t* n) {
+  return wahgpu_read_values_text_threads(path, 0, values, n);
+}
+
+// The generator's text format (src/generate_test_data.cpp:48-54): the values
+// in decimal separated by sep, then a newline.  Threads format spans of
+// values into their own buffers, which are written in order.
+int wahgpu_write_values_text(const char* path, const uint32_t* values, uint64_t n,
+                             const char* sep, int threads) {
+  if (!path || (n && !values) || !sep || threads < 0) return WAHGPU_EINVAL;
+  const size_t sl = strlen(sep);
+  FILE* f = fopen(path, "wb");
+  if (!f) return WAHGPU_EIO;
+  unsigned T = auto_threads(threads);
+  if (threads == 0 && n < ((uint64_t)1 << 16)) T = 1;
+  const uint64_t per = (n + T - 1) / (T ? T : 1);
+  std::vector<std::vector<char>> text(T);
+  auto work = [&](unsigned i) {
+    const uint64_t a = std::min<uint64_t>(n, per * i), b = std::min<uint64_t>(n, a + per);
+    std::vector<char>& o = text[i];
+    o.resize((b - a) * (10 + sl) + 1);
+    char* w = o.data();
+    for (uint64_t k = a; k < b; ++k) {
+      w += format_u32(values[k], w);
+      if (k + 1 < n) {
+        memcpy(w, sep, sl);
+        w += sl;
+      }
+    }
+    o.resize((size_t)(w - o.data()));
+  };
+  std::vector<std::thread> th;
+  for (unsigned i = 1; i < T; ++i) th.emplace_back(work, i);
+  work(0);
+  for (auto& t : th) t.join();
+  bool ok = true;
+  for (auto& o : text)
+    if (!o.empty()) ok &= fwrite(o.data(), 1, o.size(), f) == o.size();
+  ok &= fputc('\n', f) != EOF;
+  ok &= fclose(f) == 0;
+  return ok ? WAHGPU_OK : WAHGPU_EIO;
 }
 
 int wahgpu_read_values_binary(const char* path, uint32_t** values, uint64_t* n) {

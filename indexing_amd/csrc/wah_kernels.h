@@ -233,4 +233,17 @@ void launch_colen(const ScanArgs& a, const uint32_t* vals, const uint32_t* fills
 void launch_scan(const ScanArgs& a, const uint32_t* in, const uint32_t* heads, uint64_t n,
                  uint32_t* out, bool segmented, hipStream_t s);
 
+// Query side (wah_query.hip): WAH streams -> row bitmaps -> WAH streams.
+void launch_decode(const ScanArgs& s, const uint32_t* words, const uint32_t* heads,
+                   uint64_t len, uint32_t* bitmap, uint64_t nrows, uint64_t* last_row,
+                   hipStream_t st);
+void launch_mark_heads(const uint32_t* offsets, uint64_t a, uint64_t b, uint64_t w0,
+                       uint32_t* heads, hipStream_t st);
+void launch_key_range(const uint32_t* keys, const uint32_t* offsets, const uint32_t* lengths,
+                      uint64_t keycnt, uint32_t lo, uint32_t hi, uint64_t* range, hipStream_t st);
+uint64_t encode_tiles(uint64_t nrows);
+void launch_encode(const uint32_t* a, const uint32_t* b, int op, uint64_t nrows,
+                   uint64_t* tile_sum, uint32_t* out, uint64_t cap, uint64_t* len,
+                   hipStream_t st);
+
 }  // namespace wah

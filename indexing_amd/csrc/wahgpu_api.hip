@@ -1270,4 +1270,142 @@ int wahgpu_segmented_inclusive_scan(wahgpu_ctx* c, const uint32_t* d_in, const u
   return WAHGPU_OK;
 }
 
+// ---- query side: decode, range query, encode, combine --------------------
+
+namespace {
+constexpr uint64_t kMaxRows = 1ull << 32;  // rows are uint32 (src/cpu.cpp:182)
+uint64_t bitmap_words(uint64_t nrows) { return (nrows + 31) / 32; }
+}  // namespace
+
+int wahgpu_decode_device(wahgpu_ctx* c, const uint32_t* d_words, uint64_t len, uint64_t nrows,
+                         uint32_t* d_bitmap, uint64_t* last_row) {
+  if (!c) return WAHGPU_EINVAL;
+  if ((len && !d_words) || (nrows && !d_bitmap)) return fail(c, WAHGPU_EINVAL, "null buffer");
+  if (nrows > kMaxRows || len > ((uint64_t)1 << 32))
+    return fail(c, WAHGPU_ERANGE, "more than 2^32 rows or words");
+  HIPCHK(c, hipSetDevice(c->device));
+  int rc = arena_reserve(c, scan_scratch_bytes(len) + kAlign);
+  if (rc) return rc;
+  ScanScratch sc = carve_scan(c, len);
+  uint64_t* d_last = take<uint64_t>(c, 2);
+  wah::launch_fill32(sc.base, sc.words, 0, c->stream);
+  wah::launch_fill32(reinterpret_cast<uint32_t*>(d_last), 4, 0, c->stream);
+  if (nrows) wah::launch_fill32(d_bitmap, bitmap_words(nrows), 0, c->stream);
+  wah::launch_decode(sc.args, d_words, nullptr, len, d_bitmap, nrows, d_last, c->stream);
+  LAUNCHCHK(c);
+  uint64_t unused;
+  rc = read_scan_result(c, sc.header, &unused);  // fault word (synchronizes)
+  if (rc) return rc;
+  if (last_row) {
+    HIPCHK(c, hipMemcpyAsync(c->pinned_counts, d_last, 8, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    memcpy(last_row, c->pinned_counts, 8);
+  }
+  return WAHGPU_OK;
+}
+
+int wahgpu_query_range_device(wahgpu_ctx* c, const uint32_t* d_keys, const uint32_t* d_offsets,
+                              const uint32_t* d_lengths, uint64_t keycnt,
+                              const uint32_t* d_words, uint32_t lo, uint32_t hi, uint64_t nrows,
+                              uint32_t* d_bitmap, uint64_t* keys_matched) {
+  if (!c) return WAHGPU_EINVAL;
+  if ((keycnt && (!d_keys || !d_offsets || !d_lengths || !d_words)) || (nrows && !d_bitmap))
+    return fail(c, WAHGPU_EINVAL, "null buffer");
+  if (nrows > kMaxRows) return fail(c, WAHGPU_ERANGE, "more than 2^32 rows");
+  HIPCHK(c, hipSetDevice(c->device));
+  if (nrows) wah::launch_fill32(d_bitmap, bitmap_words(nrows), 0, c->stream);
+  if (keys_matched) *keys_matched = 0;
+  if (!keycnt || lo > hi) return WAHGPU_OK;
+  int rc = arena_reserve(c, align_up(4 * 8));
+  if (rc) return rc;
+  uint64_t* d_range = take<uint64_t>(c, 4);
+  wah::launch_key_range(d_keys, d_offsets, d_lengths, keycnt, lo, hi, d_range, c->stream);
+  LAUNCHCHK(c);
+  uint64_t range[4];
+  HIPCHK(c, hipMemcpyAsync(c->pinned_counts, d_range, 32, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  memcpy(range, c->pinned_counts, 32);
+  if (keys_matched) *keys_matched = range[1] - range[0];
+  const uint64_t w0 = range[2], len = range[3] - range[2];
+  if (range[1] <= range[0] || len == 0) return WAHGPU_OK;
+  // the selected keys' words are contiguous: decode them as one stream whose
+  // segments restart at each key's first word
+  rc = arena_reserve(c, scan_scratch_bytes(len) + align_up(len * 4));
+  if (rc) return rc;
+  ScanScratch sc = carve_scan(c, len);
+  uint32_t* heads = take<uint32_t>(c, len);
+  wah::launch_fill32(sc.base, sc.words, 0, c->stream);
+  wah::launch_fill32(heads, len, 0, c->stream);
+  wah::launch_mark_heads(d_offsets, range[0], range[1], w0, heads, c->stream);
+  wah::launch_decode(sc.args, d_words + w0, heads, len, d_bitmap, nrows, nullptr, c->stream);
+  LAUNCHCHK(c);
+  uint64_t unused;
+  return read_scan_result(c, sc.header, &unused);
+}
+
+int wahgpu_encode_device(wahgpu_ctx* c, const uint32_t* d_bitmap, uint64_t nrows,
+                         uint32_t* d_words, uint64_t cap, uint64_t* len) {
+  if (!c || !len) return WAHGPU_EINVAL;
+  if (nrows && (!d_bitmap || (cap && !d_words))) return fail(c, WAHGPU_EINVAL, "null buffer");
+  if (nrows > kMaxRows) return fail(c, WAHGPU_ERANGE, "more than 2^32 rows");
+  HIPCHK(c, hipSetDevice(c->device));
+  *len = 0;
+  if (!nrows) return WAHGPU_OK;
+  const uint64_t tiles = wah::encode_tiles(nrows);
+  int rc = arena_reserve(c, align_up(tiles * 24) + kAlign);
+  if (rc) return rc;
+  uint64_t* tile_sum = take<uint64_t>(c, 3 * tiles);
+  uint64_t* d_len = take<uint64_t>(c, 1);
+  wah::launch_encode(d_bitmap, nullptr, 4, nrows, tile_sum, d_words, cap, d_len, c->stream);
+  LAUNCHCHK(c);
+  HIPCHK(c, hipMemcpyAsync(c->pinned_counts, d_len, 8, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  memcpy(len, c->pinned_counts, 8);
+  if (*len > cap) return fail(c, WAHGPU_EINVAL, "output capacity too small: " +
+                                                    std::to_string(*len) + " words");
+  return WAHGPU_OK;
+}
+
+int wahgpu_combine_device(wahgpu_ctx* c, int op, const uint32_t* d_a, uint64_t la,
+                          const uint32_t* d_b, uint64_t lb, uint64_t nrows, uint32_t* d_out,
+                          uint64_t cap, uint64_t* len) {
+  if (!c || !len) return WAHGPU_EINVAL;
+  if (op < WAHGPU_OP_AND || op > WAHGPU_OP_ANDNOT) return fail(c, WAHGPU_EINVAL, "unknown op");
+  if ((la && !d_a) || (lb && !d_b) || (nrows && cap && !d_out))
+    return fail(c, WAHGPU_EINVAL, "null buffer");
+  if (nrows > kMaxRows || la > ((uint64_t)1 << 32) || lb > ((uint64_t)1 << 32))
+    return fail(c, WAHGPU_ERANGE, "more than 2^32 rows or words");
+  HIPCHK(c, hipSetDevice(c->device));
+  *len = 0;
+  if (!nrows) return WAHGPU_OK;
+  const uint64_t bw = bitmap_words(nrows), tiles = wah::encode_tiles(nrows);
+  int rc = arena_reserve(c, 2 * align_up(bw * 4) + scan_scratch_bytes(la) +
+                                scan_scratch_bytes(lb) + align_up(tiles * 24) + kAlign);
+  if (rc) return rc;
+  uint32_t* bm_a = take<uint32_t>(c, bw);
+  uint32_t* bm_b = take<uint32_t>(c, bw);
+  ScanScratch sa = carve_scan(c, la), sb = carve_scan(c, lb);
+  uint64_t* tile_sum = take<uint64_t>(c, 3 * tiles);
+  uint64_t* d_len = take<uint64_t>(c, 1);
+  wah::launch_fill32(sa.base, sa.words, 0, c->stream);
+  wah::launch_fill32(sb.base, sb.words, 0, c->stream);
+  wah::launch_fill32(bm_a, bw, 0, c->stream);
+  wah::launch_fill32(bm_b, bw, 0, c->stream);
+  wah::launch_decode(sa.args, d_a, nullptr, la, bm_a, nrows, nullptr, c->stream);
+  wah::launch_decode(sb.args, d_b, nullptr, lb, bm_b, nrows, nullptr, c->stream);
+  wah::launch_encode(bm_a, bm_b, op, nrows, tile_sum, d_out, cap, d_len, c->stream);
+  LAUNCHCHK(c);
+  uint64_t unused;
+  rc = read_scan_result(c, sa.header, &unused);
+  if (rc) return rc;
+  rc = read_scan_result(c, sb.header, &unused);
+  if (rc) return rc;
+  HIPCHK(c, hipMemcpyAsync(c->pinned_counts, d_len, 8, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  memcpy(len, c->pinned_counts, 8);
+  if (*len > cap) return fail(c, WAHGPU_EINVAL, "output capacity too small: " +
+                                                    std::to_string(*len) + " words");
+  return WAHGPU_OK;
+}
+
 }  // extern "C"

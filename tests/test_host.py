@@ -4,6 +4,7 @@ generator and the exact stitching of partial indexes (the merger).
 """
 import os
 import re
+import subprocess
 
 import numpy as np
 import pytest
@@ -14,6 +15,7 @@ from oracle import oracle
 from tests.wah_defs import encode_from_bitmaps
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+BUILD = os.path.join(ROOT, "indexing_amd", "build")
 
 
 def header_functions():
@@ -144,3 +146,86 @@ def test_merger_rejects_rows_out_of_order():
     m.add(late["keys"], late["offsets"], late["words"])
     with pytest.raises(ia.WahError):
         m.add(early["keys"], early["offsets"], early["words"])
+
+
+# ---- parallel text reader / writer ------------------------------------------
+
+def py_ifstream_read(data):
+    """Independent statement of the `while (in >> uint32_t)` loop over bytes."""
+    out, i, e = [], 0, len(data)
+    ws = b" \t\n\v\f\r"
+    while True:
+        while i < e and data[i] in ws:
+            i += 1
+        if i == e:
+            return out
+        neg = False
+        if data[i] in b"+-":
+            neg = data[i] == ord("-")
+            i += 1
+        j = i
+        while j < e and 48 <= data[j] <= 57:
+            j += 1
+        if j == i:
+            return out
+        v = int(data[i:j])
+        if v > 0xFFFFFFFF:
+            return out
+        out.append((-v) & 0xFFFFFFFF if neg else v)
+        i = j
+
+
+@pytest.mark.parametrize("threads", [1, 2, 3, 7, 16])
+@pytest.mark.parametrize("text,expect", [
+    ("1\n2\n3\n", [1, 2, 3]),
+    ("12a 13", [12]),
+    ("4294967296 5", []),
+    ("+5 -1 -0", [5, 4294967295, 0]),
+    ("1 2 3 4 5 6 7 8 9 x 10 11 12", [1, 2, 3, 4, 5, 6, 7, 8, 9]),
+    ("  ", []),
+])
+def test_text_reader_threads_match_serial_semantics(tmp_path, threads, text, expect):
+    p = write(tmp_path, text)
+    assert ia.read_values_text(p, threads=threads).tolist() == expect
+
+
+def test_text_reader_random_files_every_thread_count(tmp_path):
+    rng = np.random.default_rng(17)
+    seps = [b" ", b"\n", b"\t", b"\r\n", b"  \n ", b"\v", b"\f"]
+    for trial in range(60):
+        m = int(rng.integers(0, 400))
+        toks = []
+        for _ in range(m):
+            r = rng.random()
+            if r < 0.01:
+                toks.append(b"x")                      # a failed extraction
+            elif r < 0.02:
+                toks.append(b"99999999999")            # overflow
+            elif r < 0.04:
+                toks.append(b"-" + str(int(rng.integers(0, 1000))).encode())
+            elif r < 0.05:
+                toks.append(b"7z")                     # a value, then a failure
+            else:
+                toks.append(str(int(rng.integers(0, 1 << 32))).encode())
+        data = b"".join(t + seps[int(rng.integers(0, len(seps)))] for t in toks)
+        if rng.random() < 0.3:
+            data = data.rstrip()
+        p = write(tmp_path, data, f"r{trial}.txt")
+        want = py_ifstream_read(data)
+        for threads in (1, 2, 5, 16):
+            got = ia.read_values_text(p, threads=threads).tolist()
+            assert got == want, (trial, threads)
+
+
+def test_text_writer_roundtrip_and_matches_generate_cli(tmp_path):
+    v = ia.generate(1 << 20, "uniform", seed=9, lo=0, hi=0xFFFFFFFF)
+    p = str(tmp_path / "w.txt")
+    ia.write_values_text(p, v, threads=8)
+    assert np.array_equal(ia.read_values_text(p, threads=8), v)
+    assert np.array_equal(ia.read_values_text(p, threads=1), v)
+    q = str(tmp_path / "g.txt")
+    subprocess.run([os.path.join(BUILD, "generate"), "-a", str(1 << 20), "-f", "0", "-t",
+                    str(0xFFFFFFFF), "--seed", "9", "-o", q], check=True)
+    assert open(p, "rb").read() == open(q, "rb").read()
+    ia.write_values_text(p, v[:5], sep=",", threads=1)
+    assert open(p, "rb").read() == (",".join(str(x) for x in v[:5]) + "\n").encode()

@@ -110,6 +110,24 @@ int wahgpu_build_device(wahgpu_ctx* ctx, const uint32_t* d_values, uint64_t n,
                         uint32_t* d_lengths, uint32_t* d_words,
                         wahgpu_counts* d_counts);
 
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): wahgpu_build_device with its scratch (radix ping-pong
+ * buffers, look-back state, descriptor) taken from a static device array of
+ * 112 MiB per GPU, held until the process ends, instead of the context's
+ * arena.  It calls no hipMalloc, hipFree or other allocating routine; the
+ * context (wahgpu_create, which looks the array up) and the caller's inputs
+ * and outputs are made beforehand.  Columns of at most
+ * WAHGPU_NOALLOC_MAX_VALUES values; larger ones fail with WAHGPU_ERANGE.
+ * Every call on one GPU shares that GPU's array (the build clears what it
+ * needs itself): calls that overlap on one GPU, from other streams or
+ * contexts, race and must be ordered by the caller. */
+#define WAHGPU_NOALLOC_MAX_VALUES (((uint64_t)1) << 22)
+int wahgpu_build_device_noalloc(wahgpu_ctx* ctx, const uint32_t* d_values, uint64_t n,
+                                uint32_t rid_base, uint32_t* d_keys, uint32_t* d_offsets,
+                                uint32_t* d_lengths, uint32_t* d_words,
+                                wahgpu_counts* d_counts);
+
 /* Per-kernel timing of wahgpu_build_device calls, for measurement: between
  * wahgpu_timing_begin and wahgpu_timing_end every build_device call records
  * HIP events around its kernels on the context's stream (at most max_calls

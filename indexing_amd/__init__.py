@@ -186,6 +186,26 @@ class Context:
             _ptr(words), _ptr(counts)))
         return out
 
+    def build_device_noalloc(self, d_values, rid_base=0, out=None):
+        """build_device without any device allocation by the library: the
+        scratch is a static 112 MiB device array per GPU
+        (wahgpu_build_device_noalloc; columns of at most NOALLOC_MAX_VALUES).
+        out, if given, is reused as in build_device; calls on one GPU that
+        overlap race."""
+        torch = _torch()
+        n = int(d_values.numel())
+        if out is None:
+            cap = max(n, 1)
+            out = (self._empty(cap), self._empty(cap), self._empty(cap), self._empty(2 * cap),
+                   torch.zeros(ctypes.sizeof(N.Counts) // 4, dtype=torch.int32,
+                               device=d_values.device))
+        keys, offs, lens, words, counts = out
+        self._bind_stream()
+        self._check(self._lib.wahgpu_build_device_noalloc(
+            self._h, _ptr(d_values), n, int(rid_base), _ptr(keys), _ptr(offs), _ptr(lens),
+            _ptr(words), _ptr(counts)))
+        return out
+
     @staticmethod
     def read_counts(counts_tensor):
         raw = counts_tensor.cpu().numpy().view(np.uint8).tobytes()

@@ -14,6 +14,7 @@ LIB_PATH = os.path.join(BUILD_DIR, "libwahgpu.so")
 OK = 0
 EINVAL, ENOMEM, EHIP, ERANGE, ENODEV, EIO, EBUSY = -1, -2, -3, -4, -5, -6, -7
 MAX_VALUES = (1 << 30) - 1
+NOALLOC_MAX_VALUES = 1 << 22
 TIMING_GROUPS = 7
 
 DIST_UNIFORM, DIST_ZIPF, DIST_ZIPF_RUNS = 0, 1, 2
@@ -95,6 +96,8 @@ SIGNATURES = {
     "wahgpu_rank_mode": (ctypes.c_int, [vp]),
     "wahgpu_build_device": (ctypes.c_int, [vp, vp, ctypes.c_uint64, ctypes.c_uint32, vp, vp,
                                            vp, vp, vp]),
+    "wahgpu_build_device_noalloc": (ctypes.c_int, [vp, vp, ctypes.c_uint64, ctypes.c_uint32,
+                                                   vp, vp, vp, vp, vp]),
     "wahgpu_timing_begin": (ctypes.c_int, [vp, ctypes.c_uint32]),
     "wahgpu_timing_end": (ctypes.c_int, [vp, ctypes.POINTER(ctypes.c_double),
                                          ctypes.POINTER(ctypes.c_uint32)]),

@@ -10,6 +10,7 @@
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -50,6 +51,7 @@ struct wahgpu_ctx {
   uint32_t tune_shape[WAHGPU_TUNE_BUCKETS] = {0, 0, 0, 0, 0};
   double tune_ms[WAHGPU_TUNE_BUCKETS][WAHGPU_TUNE_SHAPES] = {};
   int force_shape = -1;  // autotune runs: every column in this shape
+  char* noalloc = nullptr;  // this device's copy of the static scratch array
 };
 
 namespace {
@@ -278,6 +280,34 @@ struct Marks {
   }
 };
 
+// Scratch of wahgpu_build_device_noalloc: a static device array, one copy
+// per GPU (loaded with the code object and held until the process ends).
+// Host code never uses the name itself (on the host it is only a stand-in
+// symbol): each device's address comes from hipGetSymbolAddress, made once
+// per device with that device current.
+constexpr uint64_t kNoallocScratchBytes = 112ull << 20;
+static __device__ __align__(256) char scratch[kNoallocScratchBytes];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_noalloc_mu;
+char* g_noalloc_ptr[kMaxDevices] = {};
+
+// Called by wahgpu_create with `device` current.
+int noalloc_lookup(wahgpu_ctx* c, int device) {
+  if (device >= kMaxDevices) return WAHGPU_OK;  // the entry point then refuses
+  std::lock_guard<std::mutex> lk(g_noalloc_mu);
+  if (!g_noalloc_ptr[device]) {
+    void* p = nullptr;
+    hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p)
+      return fail(c, WAHGPU_EHIP, std::string("hipGetSymbolAddress(scratch): ") +
+                                      hipGetErrorString(e));
+    g_noalloc_ptr[device] = static_cast<char*>(p);
+  }
+  c->noalloc = g_noalloc_ptr[device];
+  return WAHGPU_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -344,6 +374,11 @@ int wahgpu_create(wahgpu_ctx** out, int device_ordinal, uint64_t arena_bytes) {
     return WAHGPU_EHIP;
   }
   c->stream = c->own_stream;
+  if (noalloc_lookup(c, device_ordinal) != WAHGPU_OK) {
+    (void)hipGetLastError();
+    wahgpu_destroy(c);
+    return WAHGPU_EHIP;
+  }
   hipDeviceProp_t prop;
   if (hipGetDeviceProperties(&prop, device_ordinal) == hipSuccess)
     c->max_resident = (uint32_t)prop.multiProcessorCount * 4u;
@@ -445,9 +480,20 @@ struct EventHook : wah::LaunchHook {
 static int build_device_impl(wahgpu_ctx* c, const uint32_t* d_values, uint64_t n,
                              uint32_t rid_base, uint32_t* d_keys, uint32_t* d_offsets,
                              uint32_t* d_lengths, uint32_t* d_words, wahgpu_counts* d_counts,
-                             Marks* marks) {
+                             Marks* marks, char* scratch_base = nullptr) {
   BuildArgs a{};
-  carve_build(c, &a, caps_one(n, shape_for(c, n)), shape_for(c, n));
+  if (scratch_base) {
+    // carve from the caller's scratch instead of the arena
+    char* const arena = c->arena;
+    const uint64_t bump = c->bump;
+    c->arena = scratch_base;
+    c->bump = 0;
+    carve_build(c, &a, caps_one(n, shape_for(c, n)), shape_for(c, n));
+    c->arena = arena;
+    c->bump = bump;
+  } else {
+    carve_build(c, &a, caps_one(n, shape_for(c, n)), shape_for(c, n));
+  }
   set_one(&a, n, rid_base);
   a.values = d_values;
   a.out_keys = d_keys;
@@ -517,6 +563,31 @@ int wahgpu_build_device(wahgpu_ctx* c, const uint32_t* d_values, uint64_t n, uin
   if (rc) return rc;
   return build_device_impl(c, d_values, n, rid_base, d_keys, d_offsets, d_lengths, d_words,
                            d_counts, nullptr);
+}
+
+int wahgpu_build_device_noalloc(wahgpu_ctx* c, const uint32_t* d_values, uint64_t n,
+                                uint32_t rid_base, uint32_t* d_keys, uint32_t* d_offsets,
+                                uint32_t* d_lengths, uint32_t* d_words,
+                                wahgpu_counts* d_counts) {
+  if (!c || !d_counts) return fail(c, WAHGPU_EINVAL, "null context or counts");
+  if (n && (!d_values || !d_keys || !d_offsets || !d_lengths || !d_words))
+    return fail(c, WAHGPU_EINVAL, "null buffer");
+  if (n > WAHGPU_NOALLOC_MAX_VALUES)
+    return fail(c, WAHGPU_ERANGE, "wahgpu_build_device_noalloc: column of " +
+                                      std::to_string(n) + " values exceeds the limit of " +
+                                      std::to_string(WAHGPU_NOALLOC_MAX_VALUES));
+  int rc = check_n(c, n, rid_base);
+  if (rc) return rc;
+  if (!c->noalloc) return fail(c, WAHGPU_EHIP, "no static scratch on this device");
+  // every tile shape fits: 24 bytes per value plus per-tile state
+  if (build_scratch_bytes(caps_one(n, shape_for(c, n))) > kNoallocScratchBytes)
+    return fail(c, WAHGPU_ERANGE, "wahgpu_build_device_noalloc: scratch of " +
+                                      std::to_string(kNoallocScratchBytes) + " bytes too small");
+  HIPCHK(c, hipSetDevice(c->device));
+  // The build zeroes the state it uses itself (k_begin, k_histogram): the
+  // array keeps the previous call's contents and nothing is cleared here.
+  return build_device_impl(c, d_values, n, rid_base, d_keys, d_offsets, d_lengths, d_words,
+                           d_counts, nullptr, c->noalloc);
 }
 
 int wahgpu_build(wahgpu_ctx* c, const uint32_t* values, uint64_t n, uint32_t rid_base,

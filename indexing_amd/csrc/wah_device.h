@@ -75,19 +75,31 @@ __device__ __forceinline__ uint64_t shfl64(uint64_t x, int src) {
 #endif
 constexpr uint32_t kTileLanes = 8;
 constexpr uint32_t kLaneStride = 64;
+//
+// With group > 1 a lane's j-th draw is tile (j / group) * group * lanes +
+// lane * group + j % group: runs of `group` consecutive tiles go to one lane.
+// Workgroups are dispatched to the 8 XCDs round robin, so lane = block % 8
+// is the XCD, and neighbouring tiles then write the cache lines they share
+// (the ends of their output runs) through one L2, which merges them.  A
+// lane's draws still ascend, so every tile below a drawn tile is drawn by a
+// running block of its lane, and look-back waits end.
 __device__ __forceinline__ uint32_t next_tile(uint32_t* counters, uint32_t lanes,
-                                              uint32_t* s_tile, uint32_t& iter) {
+                                              uint32_t* s_tile, uint32_t& iter,
+                                              uint32_t group = 1) {
 #if WAH_STATIC_TILES
   (void)counters;
   (void)lanes;
   (void)s_tile;
+  (void)group;
   __syncthreads();
   return iter++ == 0 ? blockIdx.x : 0xFFFFFFFFu;
 #else
   (void)iter;
   if (threadIdx.x == 0) {
     const uint32_t lane = blockIdx.x % lanes;
-    *s_tile = atomicAdd(&counters[lane * kLaneStride], 1u) * lanes + lane;
+    const uint32_t j = atomicAdd(&counters[lane * kLaneStride], 1u);
+    *s_tile = group == 1 ? j * lanes + lane
+                         : (j / group) * group * lanes + lane * group + j % group;
   }
   __syncthreads();
   return *s_tile;

@@ -27,6 +27,14 @@ namespace wah {
 #ifndef WAH_SORT_WAVES_PER_EU
 #define WAH_SORT_WAVES_PER_EU 4
 #endif
+// Consecutive tiles drawn by one tile-counter lane (one XCD) in the radix
+// passes and the tail (see next_tile).
+#ifndef WAH_SORT_TILE_GROUP
+#define WAH_SORT_TILE_GROUP 8
+#endif
+#ifndef WAH_TAIL_TILE_GROUP
+#define WAH_TAIL_TILE_GROUP 4
+#endif
 
 constexpr int kSortThreads = WAH_SORT_THREADS;  // threads of a radix-pass block
 constexpr int kSortItems = WAH_SORT_ITEMS;      // keys per thread in a radix pass

@@ -624,7 +624,8 @@ __global__ __launch_bounds__(THREADS, WAH_SORT_WAVES_PER_EU) void k_onesweep(
   uint32_t iter = 0;
   for (;;) {
     for (int i = t; i < WAVES * 256; i += THREADS) (&s_wcnt[0][0])[i] = 0;
-    const uint32_t tile = __builtin_amdgcn_readfirstlane(next_tile(counter, lanes, &s_tile, iter));
+    const uint32_t tile = __builtin_amdgcn_readfirstlane(
+        next_tile(counter, lanes, &s_tile, iter, WAH_SORT_TILE_GROUP));
     if (tile >= dims.sort_tiles) break;
     const uint32_t c = find_segment(&cols[0].sort_tile0, sizeof(Column) / 4, dims.ncols, tile);
     const ColState* col = &cs[c];
@@ -940,7 +941,8 @@ __global__ __launch_bounds__(THREADS, tail_waves_per_eu(ITEMS)) void k_tail(
   uint32_t iter = 0;
   for (;;) {
     const uint32_t tile = __builtin_amdgcn_readfirstlane(next_tile(
-        &hdr->tiles[RUN ? kCounterRunTail : kCounterTail][0], lanes, &s_tile, iter));
+        &hdr->tiles[RUN ? kCounterRunTail : kCounterTail][0], lanes, &s_tile, iter,
+        WAH_TAIL_TILE_GROUP));
     if (tile >= dims.tail_tiles) break;
     const uint32_t c = find_segment(&cols[0].tail_tile0, sizeof(Column) / 4, dims.ncols, tile);
     const Column cd = cols[c];

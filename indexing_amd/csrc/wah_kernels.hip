@@ -170,16 +170,26 @@ __global__ __launch_bounds__(kBlock) void k_histogram(
     const uint32_t row0 = cd.rid_base + (uint32_t)lo;
     uint32_t reps = 0;  // rows of this chunk that repeat their predecessor
     if (cnt == kHistChunk && ((uintptr_t)src % 16) == 0) {
-      // four 16-byte loads per thread
+      // four 16-byte loads per thread, and per wave the value before each
+      // of its four 256-value spans (a uniform load), all issued before any
+      // is used: one memory latency per chunk
       const uint4* v4 = reinterpret_cast<const uint4*>(src);
+      const uint32_t wv = __builtin_amdgcn_readfirstlane(t / kWave);
       uint4 q[4];
+      uint32_t wprev[4];
 #pragma unroll
       for (int u = 0; u < 4; ++u) q[u] = v4[u * kBlock + t];
 #pragma unroll
       for (int u = 0; u < 4; ++u) {
+        const uint32_t s0 = (uint32_t)(u * kBlock + wv * kWave) * 4;  // wave's first index
+        wprev[u] = lo + s0 > 0 ? src[(int64_t)s0 - 1] : 0u;
+      }
+      __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
         const uint32_t i0 = (uint32_t)(u * kBlock + t) * 4;  // index of q[u].x in src
         uint32_t prev = (uint32_t)__shfl_up((int)q[u].w, 1, kWave);
-        if (lane_id() == 0) prev = (lo + i0 > 0) ? src[(int64_t)i0 - 1] : 0u;
+        if (lane_id() == 0) prev = wprev[u];
         const uint32_t e[4] = {q[u].x, q[u].y, q[u].z, q[u].w};
         bool rep[4];
         uint32_t m = (row0 + i0) % 31u;  // row % 31 of e[0]
@@ -958,19 +968,46 @@ __global__ __launch_bounds__(THREADS, tail_waves_per_eu(ITEMS)) void k_tail(
     }
     // the tile's body, compiled once per key width (16-bit keys load 8 per
     // 16 bytes); the column's width picks one
-    auto body = [&](auto narrow_tag) {
+    // and once for whole aligned tiles (16-byte loads, no per-item checks,
+    // and no control-flow merge with the per-item loads of partial tiles)
+    const SortedView sv0 =
+        sorted_view(&cs[c], cd, values, keysA, ridsA, keysB, ridsB, litsA, litsB);
+    const bool aligned = (((uintptr_t)sv0.keys | (uintptr_t)sv0.rids |
+                           (RUN ? (uintptr_t)sv0.lits : (uintptr_t)0)) % 16) == 0;
+    auto body = [&](auto narrow_tag, auto full_tag) {
     constexpr bool NARROW = decltype(narrow_tag)::value;
+    constexpr bool FULL = decltype(full_tag)::value;
     const uint32_t rid_base = cd.rid_base;
-    SortedView sv = sorted_view(&cs[c], cd, values, keysA, ridsA, keysB, ridsB, litsA, litsB);
+    SortedView sv = sv0;
     sv.narrow = NARROW;
     const uint32_t j0 = base + (uint32_t)t * ITEMS;  // the lane's first item
-    const uint32_t nv = n > j0 ? (n - j0 < (uint32_t)ITEMS ? n - j0 : (uint32_t)ITEMS) : 0u;
-    const bool aligned = (((uintptr_t)sv.keys | (uintptr_t)sv.rids |
-                           (RUN ? (uintptr_t)sv.lits : (uintptr_t)0)) % 16) == 0;
+    const uint32_t nv =
+        FULL ? (uint32_t)ITEMS
+             : (n > j0 ? (n - j0 < (uint32_t)ITEMS ? n - j0 : (uint32_t)ITEMS) : 0u);
+
+    // Loads the tile needs besides its own items, issued before them so that
+    // one memory latency covers all: the item before the wave (lane 0's
+    // predecessor; wave-uniform) and, for the last wave, the 32 items after
+    // the tile (the run that crosses its end).
+    // Branch-free: out-of-range indices read item 0 and are masked later
+    // (an implicit-rid column's rids array is the unused ping-pong buffer).
+    const uint32_t gw = base + (uint32_t)(w * kWave * ITEMS);
+    const bool have_pred = gw > 0 && gw <= n;
+    const uint32_t gp = have_pred ? gw - 1 : 0u;
+    uint32_t k0 = sv.key(gp), r0 = sv.rids[gp];
+    const uint32_t gn0 = base + (uint32_t)TILE + (uint32_t)l;
+    const bool nv_next = w == WAVES - 1 && l < 32 && gn0 < n;
+    const uint32_t gn = nv_next ? gn0 : 0u;
+    uint32_t kk = 0, rr = 0, ll = 0;
+    if (w == WAVES - 1) {
+      kk = sv.key(gn);
+      rr = sv.rids[gn];
+      if (RUN) ll = sv.lits[gn];
+    }
 
     // ---- items of the lane: key, chid, literal bits (0 past the column)
     uint32_t key[ITEMS], ch[ITEMS], b[ITEMS];
-    if (base + TILE <= n && aligned) {
+    if constexpr (FULL) {
       if (NARROW) {  // 8 keys per 16-byte load
 #pragma unroll
         for (int q = 0; q < ITEMS / 8; ++q) {
@@ -989,13 +1026,10 @@ __global__ __launch_bounds__(THREADS, tail_waves_per_eu(ITEMS)) void k_tail(
           const uint4 k4 = reinterpret_cast<const uint4*>(sv.keys + j0)[q];
           key[4 * q] = k4.x; key[4 * q + 1] = k4.y; key[4 * q + 2] = k4.z; key[4 * q + 3] = k4.w;
         }
-        if (sv.implicit) {
-          const uint32_t r0 = rid_base + j0 + 4u * q;
-          ch[4 * q] = r0; ch[4 * q + 1] = r0 + 1; ch[4 * q + 2] = r0 + 2; ch[4 * q + 3] = r0 + 3;
-        } else {
-          const uint4 r4 = reinterpret_cast<const uint4*>(sv.rids + j0)[q];
-          ch[4 * q] = r4.x; ch[4 * q + 1] = r4.y; ch[4 * q + 2] = r4.z; ch[4 * q + 3] = r4.w;
-        }
+        // loaded either way (an implicit-rid column reads the unused
+        // buffer), so the loads are not split by a branch
+        const uint4 r4 = reinterpret_cast<const uint4*>(sv.rids + j0)[q];
+        ch[4 * q] = r4.x; ch[4 * q + 1] = r4.y; ch[4 * q + 2] = r4.z; ch[4 * q + 3] = r4.w;
         if (RUN) {
           const uint4 l4 = reinterpret_cast<const uint4*>(sv.lits + j0)[q];
           b[4 * q] = l4.x; b[4 * q + 1] = l4.y; b[4 * q + 2] = l4.z; b[4 * q + 3] = l4.w;
@@ -1011,6 +1045,10 @@ __global__ __launch_bounds__(THREADS, tail_waves_per_eu(ITEMS)) void k_tail(
         if (RUN) b[i] = v ? sv.lits[g] : 0u;
       }
     }
+    if (FULL && sv.implicit) {
+#pragma unroll
+      for (int i = 0; i < ITEMS; ++i) ch[i] = rid_base + j0 + (uint32_t)i;
+    }
 #pragma unroll
     for (int i = 0; i < ITEMS; ++i) {  // ch[] held rids so far
       const uint32_t r = ch[i];
@@ -1021,12 +1059,14 @@ __global__ __launch_bounds__(THREADS, tail_waves_per_eu(ITEMS)) void k_tail(
     // the item before the wave (one load per wave)
     uint32_t pk, pc;
     {
-      const uint32_t gw = base + (uint32_t)(w * kWave * ITEMS);
-      uint32_t k0 = 0, c0 = 0;
-      if (gw > 0 && gw <= n) {
-        k0 = sv.key(gw - 1);
-        c0 = (sv.implicit ? rid_base + (gw - 1) : sv.rids[gw - 1]) / 31u;
-      }
+      // first use of the side loads, tied to the tile's own (so that no
+      // arithmetic on them is placed, with its wait, before those loads)
+      k0 = __builtin_amdgcn_readfirstlane(k0);
+      r0 = __builtin_amdgcn_readfirstlane(r0);
+      asm volatile("" : "+s"(k0), "+s"(r0) : "v"(key[ITEMS - 1]), "v"(ch[ITEMS - 1]));
+      if (!have_pred) k0 = r0 = 0u;
+      else if (sv.implicit) r0 = rid_base + gp;
+      const uint32_t c0 = r0 / 31u;
       pk = (uint32_t)__builtin_amdgcn_update_dpp((int)k0, (int)key[ITEMS - 1], 0x138, 0xF, 0xF, false);
       pc = (uint32_t)__builtin_amdgcn_update_dpp((int)c0, (int)ch[ITEMS - 1], 0x138, 0xF, 0xF, false);
     }
@@ -1074,14 +1114,13 @@ __global__ __launch_bounds__(THREADS, tail_waves_per_eu(ITEMS)) void k_tail(
     }
     // next tile's lead: the run that crosses the tile end (last wave, 32 items)
     if (w == WAVES - 1) {
-      const uint32_t g = base + (uint32_t)TILE + l;
-      const bool v = l < 32 && g < n;
-      uint32_t kk = 0, cc = 0, bb = 0;
+      const bool v = nv_next;
+      if (sv.implicit) rr = rid_base + gn;
+      uint32_t cc = 0, bb = 0;
+      if (!v) kk = 0u;
       if (v) {
-        const uint32_t r = sv.implicit ? rid_base + g : sv.rids[g];
-        kk = sv.key(g);
-        cc = r / 31u;
-        bb = RUN ? sv.lits[g] : 1u << (r - cc * 31u);
+        cc = rr / 31u;
+        bb = RUN ? ll : 1u << (rr - cc * 31u);
       }
       const uint32_t kl = readlane(key[ITEMS - 1], kWave - 1), cl = readlane(ch[ITEMS - 1], kWave - 1);
       const uint32_t kq = (uint32_t)__builtin_amdgcn_update_dpp((int)kl, (int)kk, 0x138, 0xF, 0xF, false);
@@ -1142,16 +1181,10 @@ __global__ __launch_bounds__(THREADS, tail_waves_per_eu(ITEMS)) void k_tail(
         carry = st ? 0u : carry;
       }
     }
-    __syncthreads();  // words staged, prefix published
-    const uint64_t pre = s_prefix;
-    const uint32_t Wpre = (uint32_t)(pre & 0x7FFFFFFFu), Kpre = (uint32_t)(pre >> 31);
-    uint32_t* out_keys = keys_out + cd.off;
-    uint32_t* out_offsets = offsets_out + cd.off;
-    uint32_t* out_words = words_out + 2 * cd.off;
+    uint32_t kv[ITEMS];
     if (khm) {
       // key starts: the values (the lane's keys read again, all at once:
       // the lines are still in L2) and the position of each key's first word
-      uint32_t kv[ITEMS];
       if (nv == (uint32_t)ITEMS && aligned && !NARROW) {
 #pragma unroll
         for (int q = 0; q < ITEMS / 4; ++q) {
@@ -1173,6 +1206,14 @@ __global__ __launch_bounds__(THREADS, tail_waves_per_eu(ITEMS)) void k_tail(
 #pragma unroll
         for (int i = 0; i < ITEMS; ++i) kv[i] = (uint32_t)i < nv ? sv.key(j0 + i) : 0u;
       }
+    }
+    __syncthreads();  // words staged, prefix published
+    const uint64_t pre = s_prefix;
+    const uint32_t Wpre = (uint32_t)(pre & 0x7FFFFFFFu), Kpre = (uint32_t)(pre >> 31);
+    uint32_t* out_keys = keys_out + cd.off;
+    uint32_t* out_offsets = offsets_out + cd.off;
+    uint32_t* out_words = words_out + 2 * cd.off;
+    if (khm) {
       bool bad = false;
 #pragma unroll
       for (int i = 0; i < ITEMS; ++i) {
@@ -1221,8 +1262,14 @@ __global__ __launch_bounds__(THREADS, tail_waves_per_eu(ITEMS)) void k_tail(
       counts[cd.index].keycnt = (uint64_t)Kpre + (tot >> 16);
     }
     };
-    if (cs[c].narrow) body(std::true_type{});
-    else body(std::false_type{});
+    const bool full = base + TILE <= n && aligned;
+    if (cs[c].narrow) {
+      if (full) body(std::true_type{}, std::true_type{});
+      else body(std::true_type{}, std::false_type{});
+    } else {
+      if (full) body(std::false_type{}, std::true_type{});
+      else body(std::false_type{}, std::false_type{});
+    }
     // the next tile's first barrier (in next_tile) orders these LDS reads
     // before its writes
   }

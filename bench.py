@@ -270,72 +270,245 @@ def bench_batched(ncols, seed=7, reps=3):
             "graph_replays": replays, "host_generation_s": round(gen_s, 1)}
 
 
-POISSON_REPS = 3  # passes over the Poisson arrivals, latencies pooled for p50/p99
+def pcie_link(nbytes=1 << 30, reps=3):
+    """Host-link rates with page-locked buffers: host-to-device, device-to-
+    host, and both at once on two streams (GB/s, best of reps)."""
+    import torch
+    h_in = torch.empty(nbytes // 4, dtype=torch.int32).pin_memory()
+    h_out = torch.empty(nbytes // 4, dtype=torch.int32).pin_memory()
+    d_in = torch.empty(nbytes // 4, dtype=torch.int32, device="cuda:0")
+    d_out = torch.empty(nbytes // 4, dtype=torch.int32, device="cuda:0")
+    s1, s2 = torch.cuda.Stream(), torch.cuda.Stream()
+    res = {}
+    for name in ("h2d", "d2h", "both"):
+        best = 0.0
+        for _ in range(reps):
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            if name in ("h2d", "both"):
+                with torch.cuda.stream(s1):
+                    d_in.copy_(h_in, non_blocking=True)
+            if name in ("d2h", "both"):
+                with torch.cuda.stream(s2):
+                    h_out.copy_(d_out, non_blocking=True)
+            torch.cuda.synchronize()
+            dt = time.perf_counter() - t0
+            best = max(best, (2 if name == "both" else 1) * nbytes / dt / 1e9)
+        res[name + "_GBs"] = round(best, 1)
+    return res
 
 
-def bench_serve(nreq, seed=11, arena_gib=64):
-    """C5: continuous-batching server vs plain per-column path on host
-    requests; burst throughput, then Poisson arrivals at a fixed rate."""
+class _ResultRing:
+    """Result buffers carved in order from one page-locked block and freed
+    in the same order (requests are retired in submission order)."""
+
+    def __init__(self, words):
+        import collections
+        import threading
+
+        import indexing_amd as ia
+        self.buf = ia.pinned_empty(words)
+        self.cap = words
+        self.live = collections.deque()  # (start, end) in allocation order
+        self.cv = threading.Condition()
+
+    def alloc(self, size):
+        with self.cv:
+            while True:
+                start = None
+                if not self.live:
+                    start = 0
+                else:
+                    tail, head = self.live[0][0], self.live[-1][1]
+                    if head > tail:      # in use: [tail, head)
+                        if head + size <= self.cap:
+                            start = head
+                        elif size <= tail:
+                            start = 0
+                    elif head + size <= tail:  # in use: [tail, cap) and [0, head)
+                        start = head
+                if start is not None:
+                    self.live.append((start, start + size))
+                    return self.buf[start:start + size]
+                self.cv.wait()
+
+    def free_oldest(self):
+        with self.cv:
+            self.live.popleft()
+            self.cv.notify_all()
+
+
+def serve_run(srv, arrivals, req_col, cols, layout, ring, pool, checksum, alone, record):
+    """Submits request i (column req_col[i]) at arrivals[i] seconds (open
+    loop) with its result buffers carved from `ring`; a retire thread waits
+    for the requests in order, checksums each result on `pool` and frees ring
+    space in order as the checksums end (and all of it before it sleeps, so a
+    submitter waiting for ring space always gets it).  Results are compared
+    with alone[c] (recorded first when `record`).  Returns the server's stats
+    and the mismatches."""
+    import collections
+    import threading
+    srv.reset_stats()
+    pend = []
+    cv = threading.Condition()
+    state = {"done": False, "late": 0.0, "out_bytes": 0}
+    sums = [None] * len(arrivals)
+
+    def retire():
+        fifo = collections.deque()
+
+        def free_front(block):
+            while fifo and (block or fifo[0][1].done()):
+                i, f = fifo.popleft()
+                sums[i] = f.result()
+                ring.free_oldest()
+
+        j = 0
+        while True:
+            with cv:
+                ready = j < len(pend) or state["done"]
+            if not ready:
+                free_front(True)
+            with cv:
+                while j >= len(pend) and not state["done"]:
+                    cv.wait()
+                if j >= len(pend):
+                    break
+                p = pend[j]
+            try:
+                srv.wait(p)
+            except Exception as e:  # a failed request is a mismatch, the run goes on
+                state.setdefault("errors", []).append(str(e))
+                fifo.append((j, pool.submit(lambda: None)))
+                free_front(False)
+                j += 1
+                continue
+            r = p.res.counts
+            K, L = int(r.keycnt), int(r.length)
+            state["out_bytes"] += 4 * L + 12 * K
+            k, o, ln, w = p.bufs
+            fifo.append((j, pool.submit(checksum, k[:K], o[:K], ln[:K], w[:L])))
+            free_front(False)
+            j += 1
+        free_front(True)
+
+    th = threading.Thread(target=retire)
+    th.start()
+    t0 = time.perf_counter()
+    for c, at in zip(req_col, arrivals):
+        delay = t0 + at - time.perf_counter()
+        if delay > 0:
+            time.sleep(delay)
+        else:
+            state["late"] = max(state["late"], -delay)
+        n = layout[c][1]
+        out = ring.alloc(5 * n + 4)
+        bufs = (out[:n], out[n:2 * n], out[2 * n:3 * n], out[3 * n:5 * n])
+        p = srv.submit(cols[c], out=bufs)
+        with cv:
+            pend.append(p)
+            cv.notify()
+    with cv:
+        state["done"] = True
+        cv.notify()
+    th.join()
+    bad = 0
+    for i, c in enumerate(req_col[:len(arrivals)]):
+        if record and alone[c] is None:
+            alone[c] = sums[i]
+        elif alone[c] != sums[i]:
+            bad += 1
+    st = srv.stats()
+    return {"p50_ms": round(st["p50_us"] / 1e3, 3), "p99_ms": round(st["p99_us"] / 1e3, 3),
+            "gvalues_s": round(st["values"] / st["busy_s"] / 1e9, 3) if st["busy_s"] else 0.0,
+            "requests": int(st["requests"]), "batches": int(st["batches"]),
+            "max_batch_cols": int(st["max_batch_cols"]),
+            "memory_waits": int(st["memory_waits"]),
+            "arena_peak_gib": round(st["arena_peak"] / 2**30, 2),
+            "max_submit_lateness_ms": round(state["late"] * 1e3, 2),
+            "out_bytes_per_value": round(state["out_bytes"] / max(1, st["values"]), 3),
+            "mismatches": bad, "errors": state.get("errors", [])[:3]}
+
+
+def bench_serve(nreq, pool_cols=512, seed=11, arena_gib=64, slots=2):
+    """C5: continuous-batching server against the plain per-column path.
+
+    Requests: nreq columns cycling over a pool of pool_cols distinct columns
+    drawn from the C4 distributions (seed 11), page-locked host memory in and
+    out.  Every served result's checksum is compared with the same column
+    built alone (the plain path).  Runs: a burst (saturation throughput),
+    open-loop Poisson arrivals at a fixed rate below the plain path's
+    saturation, and at a rate above it."""
+    import concurrent.futures as cf
+
     import numpy as np
 
     import indexing_amd as ia
     from indexing_amd import workloads as W
-    specs = W.column_specs(nreq, seed)
-    progress(f"serve: generating {nreq} requests")
-    # requests and results live in page-locked host memory (both paths)
+    specs = W.column_specs(pool_cols, seed)
+    progress(f"serve: generating a pool of {pool_cols} columns")
     buf, layout = W.make_columns(specs, alloc=ia.pinned_empty)
     cols = [buf[o:o + n] for o, n in layout]
-    total = int(sum(n for _, n in layout))
-    span = buf.size
-    okeys, ooffs, olens, owords = (ia.pinned_empty(span), ia.pinned_empty(span),
-                                   ia.pinned_empty(span), ia.pinned_empty(2 * span))
-    outs = [(okeys[o:o + n], ooffs[o:o + n], olens[o:o + n], owords[2 * o:2 * (o + n)])
-            for o, n in layout]
+    req_col = [i % pool_cols for i in range(nreq)]
+    total = int(sum(layout[c][1] for c in req_col))
+    ring = _ResultRing(1 << 30)
+    alone = [None] * pool_cols     # checksum of each column built alone (plain)
+    pool = cf.ThreadPoolExecutor(12)
 
-    def run(srv, arrivals, reps=1):
-        # latencies of `reps` passes over the same arrivals are pooled
-        progress(f"serve: {len(arrivals)} requests x {reps}")
-        srv.reset_stats()
-        for _ in range(reps):
-            t0 = time.perf_counter()
-            pend = []
-            for c, o, at in zip(cols, outs, arrivals):
-                delay = t0 + at - time.perf_counter()
-                if delay > 0:
-                    time.sleep(delay)
-                pend.append(srv.submit(c, out=o))
-            for p in pend:
-                srv.wait(p)
-        st = srv.stats()
-        return {"p50_ms": round(st["p50_us"] / 1e3, 3), "p99_ms": round(st["p99_us"] / 1e3, 3),
-                "gvalues_s": round(st["values"] / st["busy_s"] / 1e9, 3),
-                "batches": int(st["batches"]), "max_batch_cols": int(st["max_batch_cols"])}
+    def run(srv, arrivals, record):
+        return serve_run(srv, arrivals, req_col, cols, layout, ring, pool, ia.index_checksum,
+                         alone, record)
 
-    burst = [0.0] * nreq
     res = {}
-    with ia.Server(slots=1, arena_bytes=8 << 30, plain=True) as plain:
-        run(plain, burst[:8])  # warm-up (allocations)
-        res["plain_burst"] = run(plain, burst)
-    # batches of at most 2^24 values (one largest request), so that the copies
-    # of one slot overlap the build and copies of the other and no request
-    # waits behind a long batch
-    with ia.Server(slots=2, arena_bytes=arena_gib << 30, max_batch_cols=1024,
-                   max_batch_values=1 << 24) as srv:
-        run(srv, burst[:8])
-        res["served_burst"] = run(srv, burst)
-        rate = res["plain_burst"]["gvalues_s"] * 1e9 * 0.5 / (total / nreq)  # requests/s
-        gaps = np.random.default_rng(seed).exponential(1.0 / rate, nreq)
+    burst = np.zeros(nreq)
+    plain_cfg = dict(slots=1, arena_bytes=8 << 30, plain=True)
+    served_cfg = dict(slots=slots, arena_bytes=arena_gib << 30, max_batch_cols=1024,
+                      max_batch_values=1 << 24)
+    progress(f"serve: plain burst of {nreq}")
+    with ia.Server(**plain_cfg) as plain:
+        res["plain_burst"] = run(plain, burst, True)
+    progress(f"serve: served burst of {nreq}")
+    with ia.Server(**served_cfg) as srv:
+        res["served_burst"] = run(srv, burst, False)
+    rng = np.random.default_rng(seed)
+    mean_n = total / nreq
+    for name, frac in (("fixed", 0.7), ("overload", 1.25)):
+        rate = res["plain_burst"]["gvalues_s"] * 1e9 * frac / mean_n  # requests/s
+        gaps = rng.exponential(1.0 / rate, nreq)
         arrivals = np.cumsum(gaps) - gaps[0]
-        res["served_poisson"] = run(srv, arrivals, POISSON_REPS)
-    with ia.Server(slots=1, arena_bytes=8 << 30, plain=True) as plain:
-        run(plain, burst[:8])
-        res["plain_poisson"] = run(plain, arrivals, POISSON_REPS)
-    res.update({"requests": nreq, "values": total, "rate_req_s": round(rate, 1),
-                "workload": f"{nreq} requests, lengths log-uniform [2^12, 2^24], cardinality "
-                            "log-uniform [2, 2^16], half uniform half Zipf(1.0), seed 11, "
-                            f"page-locked host memory in and out, {arena_gib} GiB arena, 2 slots, "
-                            "batches of at most 2^24 values; Poisson p50/p99 over "
-                            f"{POISSON_REPS} passes of the arrivals",
+        progress(f"serve: Poisson {name} at {rate:.0f} req/s")
+        with ia.Server(**served_cfg) as srv:
+            res[f"served_{name}"] = run(srv, arrivals, False)
+        with ia.Server(**plain_cfg) as plain:
+            res[f"plain_{name}"] = run(plain, arrivals, False)
+        res[f"{name}_rate_req_s"] = round(rate, 1)
+        res[f"{name}_offered_gvalues_s"] = round(rate * mean_n / 1e9, 3)
+    pool.shutdown()
+    progress("serve: host link")
+    link = pcie_link()
+    # the link's ceiling for this mix: 4 bytes in and out_bytes_per_value out
+    # per value, each direction at its own rate, both directions at once
+    outb = res["plain_burst"]["out_bytes_per_value"]
+    # duplex: each direction at its own measured rate at once; shared: both
+    # directions through one link of the single-direction rate
+    ceil_dir = min(link["h2d_GBs"] / 4.0, link["d2h_GBs"] / outb)
+    ceil_shared = max(link["h2d_GBs"], link["d2h_GBs"]) / (4.0 + outb)
+    sb = res["served_burst"]["gvalues_s"]
+    res.update({"requests": nreq, "values": total, "pool_columns": pool_cols,
+                "link": link, "link_bytes_per_value": round(4.0 + outb, 3),
+                "link_ceiling_duplex_gvalues_s": round(ceil_dir, 3),
+                "link_ceiling_shared_gvalues_s": round(ceil_shared, 3),
+                "served_burst_link_GBs": round(sb * (4.0 + outb), 1),
+                "served_burst_of_duplex_ceiling": round(sb / ceil_dir, 3),
+                "plain_burst_of_duplex_ceiling": round(res["plain_burst"]["gvalues_s"] / ceil_dir,
+                                                       3),
+                "workload": f"{nreq} requests cycling over {pool_cols} distinct columns drawn "
+                            "from the C4 distributions (lengths log-uniform [2^12, 2^24], "
+                            "cardinality log-uniform [2, 2^16], half uniform half Zipf(1.0)), "
+                            f"seed 11, page-locked host memory in and out, one {arena_gib} GiB "
+                            f"device arena shared by {slots} slots, admission by device footprint, "
+                            "batches of at most 2^24 values; every result checked against the "
+                            "same column built alone (the plain path)",
                 "throughput_ratio_burst": round(res["served_burst"]["gvalues_s"] /
                                                 res["plain_burst"]["gvalues_s"], 2)})
     return res
@@ -400,8 +573,9 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--batch-cols", type=int, default=1024,
                     help="columns of the batched workload (0 skips it)")
-    ap.add_argument("--serve-requests", type=int, default=400,
+    ap.add_argument("--serve-requests", type=int, default=20000,
                     help="requests of the serving workload (0 skips it)")
+    ap.add_argument("--serve-slots", type=int, default=2, help="server slots (batches in flight)")
     ap.add_argument("--no-ingest", dest="ingest", action="store_false",
                     help="skip the text value-file write / parse timing")
     ap.add_argument("--cpu-baseline-worker", nargs=2, type=int, metavar=("N", "CORE"),
@@ -550,7 +724,7 @@ def main():
     if args.batch_cols:
         result["batched"] = bench_batched(args.batch_cols)
     if args.serve_requests:
-        result["serve"] = bench_serve(args.serve_requests)
+        result["serve"] = bench_serve(args.serve_requests, slots=args.serve_slots)
     if cpu is not None:
         progress("waiting for the cpu baseline")
         result["cpu_baseline"] = finish_cpu_baseline(cpu[1], cpu[0],

@@ -235,23 +235,29 @@ int wahgpu_set_tuning(wahgpu_ctx* ctx, const wahgpu_tuning* in);
 /* ---- serving: continuous batching of build requests ----------------------
  * A host request loop in place of the indexer -> worker round trips
  * (src/gpu.cpp:194-261): requests (one column each, host memory) queue as
- * they arrive; `slots` worker threads, each with its own stream, device
- * buffers and captured graph, take every queued request that fits their
- * batch (admission by HBM footprint: 49 bytes per value plus ~36 KiB per
- * column, within arena_bytes / slots) and run it as one batched build, so
- * requests arriving while others are on the GPU join the next batch of the
+ * they arrive.  The server allocates one device arena of arena_bytes at
+ * creation and carves it into per-batch blocks.  `slots` worker threads, each
+ * with its own stream, take the queued requests, in arrival order, whose
+ * combined device footprint (values, keys, offsets, lengths, words and the
+ * batched build's scratch: about 49 bytes per value plus ~37 KiB per column)
+ * fits the largest free block, within max_batch_cols columns and
+ * max_batch_values values, and build them as one batched build in that block.
+ * When the first queued request does not fit the free memory, it waits until a
+ * running batch ends and frees its block (admission by HBM footprint).
+ * Requests arriving while others are on the GPU join the next batch of the
  * first slot to free up, and the copies and kernels of different slots
  * overlap on the device.  A request completes as soon as its own result
- * copies have ended, in batch order.  With plain = 1 every slot instead serves one
- * request at a time through wahgpu_build (the per-column drop-in), for
- * comparison under the same arrivals. */
+ * copies have ended, in batch order; its block is freed after the batch's last
+ * copy.  With plain = 1 every slot instead serves one request at a time
+ * through wahgpu_build (the per-column drop-in, with its own growing arena),
+ * for comparison under the same arrivals. */
 typedef struct wahgpu_server wahgpu_server;
 typedef struct wahgpu_server_config {
   int device;
   uint32_t slots;            /* worker slots (batches in flight), >= 1 */
-  uint64_t arena_bytes;      /* device memory for all slots */
+  uint64_t arena_bytes;      /* the device arena shared by all slots */
   uint32_t max_batch_cols;   /* columns per batch */
-  uint64_t max_batch_values; /* values per batch; 0 = what arena_bytes allows */
+  uint64_t max_batch_values; /* values per batch; 0 = only the arena bounds it */
   uint32_t plain;            /* 1: one wahgpu_build per request */
   uint32_t reserved;
 } wahgpu_server_config;
@@ -261,9 +267,12 @@ typedef struct wahgpu_server_stats {
   uint64_t batches;
   uint64_t values;        /* values of completed requests */
   uint64_t max_batch_cols;
-  uint64_t slot_values;   /* admission limit per batch, in values */
+  uint64_t slot_values;   /* max_batch_values of the configuration */
   double p50_us, p99_us, mean_us, max_us;  /* submit -> result in host buffers */
   double busy_s;          /* first submit -> last completion */
+  uint64_t memory_waits;  /* times a slot waited for arena memory to free up */
+  uint64_t arena_bytes;   /* the device arena */
+  uint64_t arena_peak;    /* most bytes of it in use at once */
 } wahgpu_server_stats;
 
 int wahgpu_server_create(wahgpu_server** out, const wahgpu_server_config* cfg);
@@ -271,8 +280,9 @@ int wahgpu_server_create(wahgpu_server** out, const wahgpu_server_config* cfg);
 void wahgpu_server_destroy(wahgpu_server* s);
 /* Queues one column.  values must stay valid and the result buffers
  * (capacities as for wahgpu_build) untouched until the request completes;
- * result->counts is filled then.  Fails with WAHGPU_ERANGE for a column that
- * exceeds a slot.  *id names the request. */
+ * result->counts is filled then.  Fails with WAHGPU_ERANGE for a column whose
+ * footprint exceeds the whole arena (or max_batch_values).  *id names the
+ * request. */
 int wahgpu_server_submit(wahgpu_server* s, const uint32_t* values, uint64_t n,
                          uint32_t rid_base, wahgpu_result* result, uint64_t* id);
 /* WAHGPU_OK when request id has completed, WAHGPU_EBUSY while pending,
@@ -281,6 +291,9 @@ int wahgpu_server_poll(wahgpu_server* s, uint64_t id);
 /* Blocks until request id completes; returns as poll.  Completed requests
  * are forgotten once polled or waited for with a final status. */
 int wahgpu_server_wait(wahgpu_server* s, uint64_t id);
+/* Message of the last request that failed (empty if none); the pointer is
+ * valid in the calling thread until its next call. */
+const char* wahgpu_server_last_error(wahgpu_server* s);
 /* Blocks until every submitted request has completed. */
 int wahgpu_server_drain(wahgpu_server* s);
 int wahgpu_server_get_stats(wahgpu_server* s, wahgpu_server_stats* out);
@@ -424,6 +437,13 @@ int wahgpu_write_values_text(const char* path, const uint32_t* values, uint64_t 
 /* Raw little-endian uint32 files (opt-in extension). */
 int wahgpu_read_values_binary(const char* path, uint32_t** values, uint64_t* n);
 void wahgpu_free(void* p);
+/* 64-bit order-sensitive checksum of an index (its keycnt keys, offsets and
+ * lengths and its `length` words): equal indexes give equal checksums, so a
+ * result can be compared with another build of the same column without
+ * keeping both (e.g. a served request against the same column built alone). */
+uint64_t wahgpu_index_checksum(const uint32_t* keys, const uint32_t* offsets,
+                               const uint32_t* lengths, uint64_t keycnt, const uint32_t* words,
+                               uint64_t length);
 /* src/cpu.cpp:47-74: '0'/'1' text of a key's bitmap, trailing '0's trimmed.
  * Writes at most cap bytes (no terminator); returns the full length. */
 uint64_t wahgpu_decode_bitmap(const uint32_t* words, uint64_t len, char* out,

@@ -22,7 +22,7 @@ from ._native import WahError, MAX_VALUES
 
 __all__ = [
     "Context", "Server", "Index", "pinned_empty", "WahError", "MAX_VALUES", "Merger", "read_values_text",
-    "write_values_text",
+    "write_values_text", "index_checksum",
     "read_values_binary", "decode_bitmap", "generate", "device_count",
 ]
 
@@ -477,6 +477,18 @@ def read_values_binary(path):
     return _read(N.lib().wahgpu_read_values_binary, path)
 
 
+def index_checksum(keys, offsets, lengths, words):
+    """64-bit order-sensitive checksum of an index (wahgpu_index_checksum):
+    equal for equal keys, offsets, lengths and words.  Runs without the GIL."""
+    k, o, ln, w = (_as_u32(a) for a in (keys, offsets, lengths, words))
+    if not (k.size == o.size == ln.size):
+        raise ValueError("keys, offsets and lengths differ in size")
+    return int(N.lib().wahgpu_index_checksum(k.ctypes.data_as(ctypes.c_void_p),
+                                             o.ctypes.data_as(ctypes.c_void_p),
+                                             ln.ctypes.data_as(ctypes.c_void_p), k.size,
+                                             w.ctypes.data_as(ctypes.c_void_p), w.size))
+
+
 def decode_bitmap(words):
     """'0'/'1' text of a key's bitmap, trailing zeros trimmed (src/cpu.cpp:47-74)."""
     w = _as_u32(words)
@@ -651,13 +663,17 @@ class Server:
         if rc == N.EBUSY:
             return None
         if rc != N.OK:
-            raise WahError(rc, f"request {p.id} failed")
+            self._fail(rc, p)
         return self._index(p)
+
+    def _fail(self, rc, p):
+        msg = self._lib.wahgpu_server_last_error(self._h)
+        raise WahError(rc, f"request {p.id} failed: {msg.decode() if msg else ''}")
 
     def wait(self, p):
         rc = self._lib.wahgpu_server_wait(self._h, p.id)
         if rc != N.OK:
-            raise WahError(rc, f"request {p.id} failed")
+            self._fail(rc, p)
         return self._index(p)
 
     def drain(self):

@@ -67,7 +67,9 @@ class ServerStats(ctypes.Structure):
                 ("values", ctypes.c_uint64), ("max_batch_cols", ctypes.c_uint64),
                 ("slot_values", ctypes.c_uint64), ("p50_us", ctypes.c_double),
                 ("p99_us", ctypes.c_double), ("mean_us", ctypes.c_double),
-                ("max_us", ctypes.c_double), ("busy_s", ctypes.c_double)]
+                ("max_us", ctypes.c_double), ("busy_s", ctypes.c_double),
+                ("memory_waits", ctypes.c_uint64), ("arena_bytes", ctypes.c_uint64),
+                ("arena_peak", ctypes.c_uint64)]
 
 
 TUNE_BUCKETS, TUNE_SHAPES = 5, 3
@@ -125,6 +127,7 @@ SIGNATURES = {
     "wahgpu_server_poll": (ctypes.c_int, [vp, ctypes.c_uint64]),
     "wahgpu_server_wait": (ctypes.c_int, [vp, ctypes.c_uint64]),
     "wahgpu_server_drain": (ctypes.c_int, [vp]),
+    "wahgpu_server_last_error": (ctypes.c_char_p, [vp]),
     "wahgpu_server_get_stats": (ctypes.c_int, [vp, ctypes.POINTER(ServerStats)]),
     "wahgpu_server_reset_stats": (ctypes.c_int, [vp]),
     "wahgpu_host_alloc": (ctypes.c_int, [ctypes.c_uint64, ctypes.POINTER(vp)]),
@@ -168,6 +171,8 @@ SIGNATURES = {
     "wahgpu_read_values_binary": (ctypes.c_int, [ctypes.c_char_p, ctypes.POINTER(c_u32p),
                                                  c_u64p]),
     "wahgpu_free": (None, [vp]),
+    "wahgpu_index_checksum": (ctypes.c_uint64, [vp, vp, vp, ctypes.c_uint64, vp,
+                                                ctypes.c_uint64]),
     "wahgpu_decode_bitmap": (ctypes.c_uint64, [vp, ctypes.c_uint64, ctypes.c_char_p,
                                                ctypes.c_uint64]),
     "wahgpu_generate": (ctypes.c_int, [ctypes.POINTER(GenParams), vp, ctypes.c_uint64]),

@@ -53,9 +53,14 @@ inline uint32_t below(uint64_t r, uint64_t range) {
 // Uniform double in [0, 1).
 inline double unit(uint64_t r) { return (double)(r >> 11) * (1.0 / 9007199254740992.0); }
 
-// Inverse-CDF sampler for Zipf(s) over ranks 1..card.
+// Inverse-CDF sampler for Zipf(s) over ranks 1..card.  rank0(u) is the
+// first rank whose cumulative probability exceeds u (std::upper_bound over
+// the CDF), found from a guide table (Chen & Asau, 1974): guide[g] is that
+// rank for u = g / G, and the answer for u in [g / G, (g + 1) / G) is at or
+// after it, usually within a step or two.
 struct Zipf {
   std::vector<double> cdf;
+  std::vector<uint32_t> guide;
   Zipf(uint32_t card, double s) : cdf(card) {
     double acc = 0.0;
     for (uint32_t k = 0; k < card; ++k) {
@@ -63,9 +68,21 @@ struct Zipf {
       cdf[k] = acc;
     }
     for (auto& x : cdf) x /= acc;
+    const uint32_t G = card;
+    guide.resize(G);
+    uint32_t i = 0;
+    for (uint32_t g = 0; g < G; ++g) {
+      const double u = (double)g / (double)G;
+      while (i < card && cdf[i] <= u) ++i;
+      guide[g] = i;
+    }
   }
   uint32_t rank0(double u) const {  // rank - 1
-    size_t i = std::upper_bound(cdf.begin(), cdf.end(), u) - cdf.begin();
+    const size_t G = guide.size();
+    size_t g = (size_t)(u * (double)G);
+    if (g >= G) g = G - 1;
+    size_t i = guide[g];
+    while (i < cdf.size() && cdf[i] <= u) ++i;
     return (uint32_t)std::min(i, cdf.size() - 1);
   }
 };
@@ -290,6 +307,36 @@ int wahgpu_read_values_binary(const char* path, uint32_t** values, uint64_t* n) 
 }
 
 void wahgpu_free(void* p) { free(p); }
+
+// Four interleaved multiply-xorshift chains over 64-bit words; each chain
+// depends on the order of its words, and the lengths are folded in.
+static inline uint64_t chk_mix(uint64_t h, uint64_t v) {
+  h ^= v;
+  h *= 0x9E3779B97F4A7C15ull;
+  return h ^ (h >> 29);
+}
+static uint64_t chk_array(uint64_t h, const uint32_t* p, uint64_t n) {
+  uint64_t a[4] = {h ^ 0x243F6A8885A308D3ull, h ^ 0x13198A2E03707344ull,
+                   h ^ 0xA4093822299F31D0ull, h ^ 0x082EFA98EC4E6C89ull};
+  uint64_t i = 0;
+  for (; i + 8 <= n; i += 8)
+    for (int k = 0; k < 4; ++k)
+      a[k] = chk_mix(a[k], (uint64_t)p[i + 2 * k] | ((uint64_t)p[i + 2 * k + 1] << 32));
+  for (int k = 0; i < n; ++i, k = (k + 1) & 3) a[k] = chk_mix(a[k], p[i]);
+  uint64_t r = chk_mix(n, a[0]);
+  for (int k = 1; k < 4; ++k) r = chk_mix(r, a[k]);
+  return r;
+}
+
+uint64_t wahgpu_index_checksum(const uint32_t* keys, const uint32_t* offsets,
+                               const uint32_t* lengths, uint64_t keycnt, const uint32_t* words,
+                               uint64_t length) {
+  uint64_t h = 0x6A09E667F3BCC908ull;
+  h = chk_array(h, keys, keys ? keycnt : 0);
+  h = chk_array(h, offsets, offsets ? keycnt : 0);
+  h = chk_array(h, lengths, lengths ? keycnt : 0);
+  return chk_array(h, words, words ? length : 0);
+}
 
 // src/cpu.cpp:47-74 (== src/gpu.cpp:65-94): literal words (bit 31 set) give
 // their 31 payload bits LSB first; other words repeat bit 30 for 31 x (bits

@@ -4,20 +4,24 @@
 // The reference serves index builds through actors: the indexer slices the
 // column and keeps `jobs` worker requests in flight, each a separate OpenCL
 // launch set with its own transfers (src/gpu.cpp:194-261).  Here requests
-// from any caller thread join one queue; each slot (a thread with its own
-// wahgpu_ctx, stream, device buffers and captured graph) takes every queued
-// request that fits its HBM budget and builds them as one batch.  While a
-// slot copies or builds, arriving requests wait for the next free slot, so
-// batches grow with the load (continuous batching) and the transfers and
-// kernels of different slots overlap on the device.
+// from any caller thread join one queue.  The server owns one device arena
+// (allocated once) that a host-side pool allocator carves into per-batch
+// blocks.  Each slot (a thread with its own wahgpu_ctx and stream) takes the
+// queued requests, in arrival order, whose combined device footprint (values,
+// outputs and the batched build's scratch) fits the largest free block, and
+// builds them as one batch in that block; when even the first queued request
+// does not fit, the slot waits until a running batch ends and frees its block.
+// While slots copy or build, arriving requests queue, so batches grow with
+// the load (continuous batching), and the transfers and kernels of different
+// slots overlap on the device.
 #include <hip/hip_runtime_api.h>
 #include <stdint.h>
 
 #include <algorithm>
-#include <atomic>
 #include <chrono>
 #include <condition_variable>
 #include <deque>
+#include <map>
 #include <memory>
 #include <mutex>
 #include <string>
@@ -25,22 +29,78 @@
 #include <unordered_map>
 #include <vector>
 
+#include "wah_internal.h"
 #include "wahgpu.h"
 
 namespace {
 
 using Clock = std::chrono::steady_clock;
 
-// Device bytes a batch needs per value: the values, keys, offsets, lengths
-// and words (2 per value) outputs, the two radix ping-pong buffers of
-// (key, rid, literal) items and the look-back state; and per column: its
-// state block, descriptor and tile rounding.
-constexpr uint64_t kBytesPerValue = 4 + 12 + 8 + 24 + 1;
-constexpr uint64_t kBytesPerColumn = 12544 + 32 + 3 * 8192;
-constexpr uint64_t kSlotMargin = 64ull << 20;
 constexpr uint64_t kAlignValues = 64;  // columns start on 256-byte boundaries
+constexpr uint64_t kAlignBytes = 256;
 
 uint64_t align_values(uint64_t n) { return (n + kAlignValues - 1) / kAlignValues * kAlignValues; }
+uint64_t align_bytes(uint64_t b) { return (b + kAlignBytes - 1) / kAlignBytes * kAlignBytes; }
+
+// Device bytes of a batch's inputs and outputs: values, keys, offsets and
+// lengths (V each), words (2V) and one counts record per column.
+uint64_t io_bytes(uint64_t values, uint64_t cols) {
+  const uint64_t v = values ? values : 1;  // an all-empty batch still carves one value
+  return 4 * align_bytes(v * 4) + align_bytes(2 * v * 4) +
+         align_bytes(cols * sizeof(wahgpu_counts));
+}
+
+// The whole device footprint of a batch: inputs, outputs, scratch.
+uint64_t footprint(const wah_batch_acc& acc) {
+  return io_bytes(acc.values, acc.cols) + wah_batch_acc_scratch(&acc);
+}
+
+// First-fit allocator over [0, cap) with coalescing of freed blocks.
+struct Pool {
+  uint64_t cap = 0, used = 0, peak = 0;
+  std::map<uint64_t, uint64_t> free_;  // offset -> size
+  void init(uint64_t c) {
+    cap = c;
+    free_.clear();
+    free_[0] = c;
+  }
+  uint64_t largest() const {
+    uint64_t m = 0;
+    for (const auto& f : free_) m = std::max(m, f.second);
+    return m;
+  }
+  bool alloc(uint64_t size, uint64_t* off) {
+    size = align_bytes(size);
+    for (auto it = free_.begin(); it != free_.end(); ++it) {
+      if (it->second < size) continue;
+      *off = it->first;
+      const uint64_t rest = it->second - size, at = it->first + size;
+      free_.erase(it);
+      if (rest) free_[at] = rest;
+      used += size;
+      peak = std::max(peak, used);
+      return true;
+    }
+    return false;
+  }
+  void release(uint64_t off, uint64_t size) {
+    size = align_bytes(size);
+    used -= size;
+    auto it = free_.emplace(off, size).first;
+    auto next = std::next(it);
+    if (next != free_.end() && it->first + it->second == next->first) {
+      it->second += next->second;
+      free_.erase(next);
+    }
+    if (it != free_.begin()) {
+      auto prev = std::prev(it);
+      if (prev->first + prev->second == it->first) {
+        prev->second += it->second;
+        free_.erase(it);
+      }
+    }
+  }
+};
 
 struct Request {
   uint64_t id = 0;
@@ -54,13 +114,8 @@ struct Request {
 
 struct Slot {
   wahgpu_ctx* ctx = nullptr;
-  uint32_t* d_values = nullptr;
-  uint32_t* d_keys = nullptr;
-  uint32_t* d_offsets = nullptr;
-  uint32_t* d_lengths = nullptr;
-  uint32_t* d_words = nullptr;
-  wahgpu_counts* d_counts = nullptr;
-  std::vector<wahgpu_counts> counts;
+  std::vector<wahgpu_counts> counts;  // host landing (pinned) for a batch's counts
+  wahgpu_counts* hcounts = nullptr;
   std::vector<hipEvent_t> done;  // one per batch column: its result copies ended
   std::thread th;
 };
@@ -69,10 +124,12 @@ struct Slot {
 
 struct wahgpu_server {
   wahgpu_server_config cfg{};
-  uint64_t slot_values = 0;
+  char* arena = nullptr;  // the pooled device arena (batched mode)
+  Pool pool;
   std::mutex mu;
   std::condition_variable work_cv;  // queue non-empty or stop
   std::condition_variable done_cv;  // a request completed
+  std::condition_variable mem_cv;   // a batch freed its block
   std::deque<Request*> queue;
   std::unordered_map<uint64_t, std::shared_ptr<Request>> reqs;
   uint64_t next_id = 1;
@@ -81,7 +138,8 @@ struct wahgpu_server {
   std::vector<std::unique_ptr<Slot>> slots;
   // statistics
   std::vector<double> lat_us;
-  uint64_t batches = 0, values = 0, max_cols = 0;
+  uint64_t batches = 0, values = 0, max_cols = 0, memory_waits = 0;
+  std::string last_error;  // message of the last failed request
   bool have_first = false;
   Clock::time_point first_submit, last_done;
 };
@@ -98,11 +156,13 @@ int run_plain(Slot* sl, Request* r) {
 // Completes one request: status and latency under the lock, then wakes the
 // waiters.  Called as each request's result copies end, so the small
 // requests of a batch do not wait for the copies of its large ones.
-void publish(wahgpu_server* s, Request* r, int status) {
+void publish(wahgpu_server* s, Request* r, int status, const char* why = nullptr) {
   const Clock::time_point now = Clock::now();
   {
     std::lock_guard<std::mutex> lk(s->mu);
     r->status = status;
+    if (status != WAHGPU_OK)
+      s->last_error = "request " + std::to_string(r->id) + ": " + (why ? why : "failed");
     s->lat_us.push_back(std::chrono::duration<double, std::micro>(now - r->submitted).count());
     s->values += r->n;
     --s->pending;
@@ -111,61 +171,90 @@ void publish(wahgpu_server* s, Request* r, int status) {
   s->done_cv.notify_all();
 }
 
-// One batch: copy in, batched build (graph replay), counts, then each
-// request's result copies followed by its event; requests complete in
-// batch order as their events pass.
-void run_batch(wahgpu_server* s, Slot* sl, const std::vector<Request*>& batch) {
+// One batch in the arena block [off, off + bytes): copy in, batched build
+// on the block's scratch, counts, then each request's result copies followed
+// by its event; requests complete in batch order as their events pass; the
+// block is freed once every copy out of it has ended.
+void run_batch(wahgpu_server* s, Slot* sl, const std::vector<Request*>& batch, uint64_t off,
+               uint64_t bytes) {
   hipStream_t st = (hipStream_t)wahgpu_stream(sl->ctx);
+  auto release = [&]() {
+    {
+      std::lock_guard<std::mutex> lk(s->mu);
+      s->pool.release(off, bytes);
+    }
+    s->mem_cv.notify_all();
+  };
   auto fail_all = [&](size_t from, int rc) {
     (void)hipStreamSynchronize(st);  // no copy may still target a caller's buffer
-    for (size_t i = from; i < batch.size(); ++i) publish(s, batch[i], rc);
+    const std::string why = wahgpu_last_error(sl->ctx);
+    for (size_t i = from; i < batch.size(); ++i) publish(s, batch[i], rc, why.c_str());
+    release();
   };
   std::vector<wahgpu_column> cols(batch.size());
-  uint64_t off = 0;
+  uint64_t V = 0;
   for (size_t i = 0; i < batch.size(); ++i) {
-    Request* r = batch[i];
-    cols[i] = wahgpu_column{off, r->n, r->rid_base, 0u};
-    if (r->n &&
-        hipMemcpyAsync(sl->d_values + off, r->values, r->n * 4, hipMemcpyHostToDevice, st) !=
-            hipSuccess)
-      return fail_all(0, WAHGPU_EHIP);
-    off += align_values(r->n);
+    cols[i] = wahgpu_column{V, batch[i]->n, batch[i]->rid_base, 0u};
+    V += align_values(batch[i]->n);
   }
-  // values_len is the slot capacity, so every batch replays the same graph
-  int rc = wahgpu_build_batch_device(sl->ctx, sl->d_values, s->slot_values, cols.data(),
-                                     (uint32_t)cols.size(), sl->d_keys, sl->d_offsets,
-                                     sl->d_lengths, sl->d_words, sl->d_counts);
+  const uint64_t nv = V ? V : 1;
+  char* p = s->arena + off;
+  auto take = [&](uint64_t b) {
+    char* q = p;
+    p += align_bytes(b);
+    return q;
+  };
+  uint32_t* d_values = reinterpret_cast<uint32_t*>(take(nv * 4));
+  uint32_t* d_keys = reinterpret_cast<uint32_t*>(take(nv * 4));
+  uint32_t* d_offsets = reinterpret_cast<uint32_t*>(take(nv * 4));
+  uint32_t* d_lengths = reinterpret_cast<uint32_t*>(take(nv * 4));
+  uint32_t* d_words = reinterpret_cast<uint32_t*>(take(2 * nv * 4));
+  wahgpu_counts* d_counts = reinterpret_cast<wahgpu_counts*>(take(batch.size() * sizeof(wahgpu_counts)));
+  const uint64_t io = (uint64_t)(p - (s->arena + off));
+  for (size_t i = 0; i < batch.size(); ++i) {
+    const Request* r = batch[i];
+    if (r->n && hipMemcpyAsync(d_values + cols[i].offset, r->values, r->n * 4,
+                               hipMemcpyHostToDevice, st) != hipSuccess)
+      return fail_all(0, WAHGPU_EHIP);
+  }
+  int rc = wah_build_batch_ws(sl->ctx, d_values, V, cols.data(), (uint32_t)cols.size(), d_keys,
+                              d_offsets, d_lengths, d_words, d_counts, p, bytes - io);
   if (rc != WAHGPU_OK) return fail_all(0, rc);
-  if (hipMemcpyAsync(sl->counts.data(), sl->d_counts, batch.size() * sizeof(wahgpu_counts),
+  if (hipMemcpyAsync(sl->hcounts, d_counts, batch.size() * sizeof(wahgpu_counts),
                      hipMemcpyDeviceToHost, st) != hipSuccess ||
       hipStreamSynchronize(st) != hipSuccess)
     return fail_all(0, WAHGPU_EHIP);
   std::vector<int> status(batch.size(), WAHGPU_OK);
+  std::string why;  // message of a column the batch could not return
   size_t issued = 0;
   for (; issued < batch.size(); ++issued) {
     const size_t i = issued;
     Request* r = batch[i];
-    const wahgpu_counts& k = sl->counts[i];
+    const wahgpu_counts& k = sl->hcounts[i];
     wahgpu_result* res = r->result;
     res->counts = k;
     bool ok = true;
     if (k.fault) {
       status[i] = WAHGPU_EHIP;
+      why = "kernel reported fault " + std::to_string(k.fault) +
+            " (look-back wait gave up or index out of range) in a batch of " +
+            std::to_string(batch.size());
     } else if (k.keycnt > res->keys_cap || k.length > res->words_cap ||
                (k.length && !res->words) ||
                (k.keycnt && (!res->keys || !res->offsets || !res->lengths))) {
       status[i] = WAHGPU_EINVAL;
+      why = "result capacity too small";
     } else {
       const uint64_t o = cols[i].offset;
       if (k.length)
-        ok &= hipMemcpyAsync(res->words, sl->d_words + 2 * o, k.length * 4,
-                             hipMemcpyDeviceToHost, st) == hipSuccess;
-      if (k.keycnt) {
-        ok &= hipMemcpyAsync(res->keys, sl->d_keys + o, k.keycnt * 4, hipMemcpyDeviceToHost,
+        ok &= hipMemcpyAsync(res->words, d_words + 2 * o, k.length * 4, hipMemcpyDeviceToHost,
                              st) == hipSuccess;
-        ok &= hipMemcpyAsync(res->offsets, sl->d_offsets + o, k.keycnt * 4,
+      if (k.keycnt) {
+        ok &= hipMemcpyAsync(res->keys, d_keys + o, k.keycnt * 4, hipMemcpyDeviceToHost,
+                             st) == hipSuccess;
+        ok &= hipMemcpyAsync(res->offsets, d_offsets + o, k.keycnt * 4,
                              hipMemcpyDeviceToHost, st) == hipSuccess;
-        ok &= hipMemcpyAsync(res->lengths, sl->d_lengths + o, k.keycnt * 4,
+        ok &= hipMemcpyAsync(res->lengths, d_lengths + o, k.keycnt * 4,
                              hipMemcpyDeviceToHost, st) == hipSuccess;
       }
     }
@@ -173,9 +262,10 @@ void run_batch(wahgpu_server* s, Slot* sl, const std::vector<Request*>& batch) {
   }
   for (size_t i = 0; i < issued; ++i) {
     if (hipEventSynchronize(sl->done[i]) != hipSuccess) return fail_all(i, WAHGPU_EHIP);
-    publish(s, batch[i], status[i]);
+    publish(s, batch[i], status[i], status[i] ? why.c_str() : nullptr);
   }
-  if (issued < batch.size()) fail_all(issued, WAHGPU_EHIP);
+  if (issued < batch.size()) return fail_all(issued, WAHGPU_EHIP);
+  release();
 }
 
 void slot_loop(wahgpu_server* s, Slot* sl) {
@@ -183,38 +273,74 @@ void slot_loop(wahgpu_server* s, Slot* sl) {
   std::vector<Request*> batch;
   for (;;) {
     batch.clear();
+    uint64_t off = 0, bytes = 0;
     {
       std::unique_lock<std::mutex> lk(s->mu);
-      s->work_cv.wait(lk, [&] { return s->stop || !s->queue.empty(); });
-      if (s->queue.empty()) return;  // stopping and nothing left
-      uint64_t used = 0;
-      while (!s->queue.empty() && batch.size() < s->cfg.max_batch_cols) {
-        Request* r = s->queue.front();
-        const uint64_t need = align_values(r->n);
-        if (!batch.empty() && used + need > s->slot_values) break;
-        batch.push_back(r);
-        used += need;
-        s->queue.pop_front();
-        if (s->cfg.plain) break;
+      for (;;) {
+        s->work_cv.wait(lk, [&] { return s->stop || !s->queue.empty(); });
+        if (s->queue.empty()) return;  // stopping and nothing left
+        if (s->cfg.plain) {
+          batch.push_back(s->queue.front());
+          s->queue.pop_front();
+          break;
+        }
+        // the queued requests, in order, whose footprint fits the largest
+        // free block (and the batch limits)
+        const uint64_t avail = s->pool.largest();
+        wah_batch_acc acc{};
+        uint64_t fp = 0;
+        size_t take = 0;
+        for (size_t i = 0; i < s->queue.size() && take < s->cfg.max_batch_cols; ++i) {
+          wah_batch_acc next = acc;
+          wah_batch_acc_add(&next, s->queue[i]->n);
+          if (take && s->cfg.max_batch_values && next.values > s->cfg.max_batch_values) break;
+          const uint64_t f = footprint(next);
+          if (f > avail) break;
+          acc = next;
+          fp = f;
+          ++take;
+        }
+        if (take == 0) {
+          // the first queued request needs more than is free: wait for a
+          // running batch to end (submit refused requests the empty arena
+          // cannot hold, so this wait ends)
+          ++s->memory_waits;
+          s->mem_cv.wait(lk);
+          continue;
+        }
+        if (!s->pool.alloc(fp, &off)) continue;  // cannot happen: fp <= largest
+        bytes = fp;
+        for (size_t i = 0; i < take; ++i) {
+          batch.push_back(s->queue.front());
+          s->queue.pop_front();
+        }
+        break;
       }
       ++s->batches;
       s->max_cols = std::max<uint64_t>(s->max_cols, batch.size());
     }
-    if (s->cfg.plain) publish(s, batch[0], run_plain(sl, batch[0]));
-    else run_batch(s, sl, batch);
+    if (s->cfg.plain) {
+      const int rc = run_plain(sl, batch[0]);
+      publish(s, batch[0], rc, rc ? wahgpu_last_error(sl->ctx) : nullptr);
+    }
+    else run_batch(s, sl, batch, off, bytes);
   }
 }
 
 void free_slot(Slot* sl) {
-  if (sl->d_values) (void)hipFree(sl->d_values);
-  if (sl->d_keys) (void)hipFree(sl->d_keys);
-  if (sl->d_offsets) (void)hipFree(sl->d_offsets);
-  if (sl->d_lengths) (void)hipFree(sl->d_lengths);
-  if (sl->d_words) (void)hipFree(sl->d_words);
-  if (sl->d_counts) (void)hipFree(sl->d_counts);
   for (hipEvent_t e : sl->done) (void)hipEventDestroy(e);
   sl->done.clear();
+  if (sl->hcounts) (void)hipHostFree(sl->hcounts);
+  sl->hcounts = nullptr;
   if (sl->ctx) wahgpu_destroy(sl->ctx);
+  sl->ctx = nullptr;
+}
+
+// Footprint of a batch of one column of n values.
+uint64_t single_footprint(uint64_t n) {
+  wah_batch_acc acc{};
+  wah_batch_acc_add(&acc, n);
+  return footprint(acc);
 }
 
 }  // namespace
@@ -228,29 +354,22 @@ int wahgpu_server_create(wahgpu_server** out, const wahgpu_server_config* cfg) {
   auto s = std::make_unique<wahgpu_server>();
   s->cfg = *cfg;
   if (s->cfg.plain) s->cfg.max_batch_cols = 1;
-  const uint64_t per_slot = cfg->arena_bytes / cfg->slots;
-  const uint64_t fixed = kSlotMargin + (uint64_t)s->cfg.max_batch_cols * kBytesPerColumn;
-  if (per_slot <= fixed) return WAHGPU_ENOMEM;
-  uint64_t v = (per_slot - fixed) / kBytesPerValue;
-  if (cfg->max_batch_values && cfg->max_batch_values < v) v = cfg->max_batch_values;
-  v = v / kAlignValues * kAlignValues;
-  if (v > WAHGPU_MAX_VALUES) v = (WAHGPU_MAX_VALUES / kAlignValues) * kAlignValues;
-  if (v == 0) return WAHGPU_ENOMEM;
-  s->slot_values = v;
   if (hipSetDevice(cfg->device) != hipSuccess) return WAHGPU_ENODEV;
+  if (!s->cfg.plain) {
+    if (hipMalloc(&s->arena, cfg->arena_bytes) != hipSuccess) {
+      (void)hipGetLastError();
+      return WAHGPU_ENOMEM;
+    }
+    s->pool.init(cfg->arena_bytes);
+  }
   for (uint32_t i = 0; i < cfg->slots; ++i) {
     auto sl = std::make_unique<Slot>();
     int rc = wahgpu_create(&sl->ctx, cfg->device, 0);
     bool ok = rc == WAHGPU_OK;
     if (ok && !s->cfg.plain) {
-      ok = wahgpu_batch_reserve(sl->ctx, v, s->cfg.max_batch_cols) == WAHGPU_OK &&
-           hipMalloc(&sl->d_values, v * 4) == hipSuccess &&
-           hipMalloc(&sl->d_keys, v * 4) == hipSuccess &&
-           hipMalloc(&sl->d_offsets, v * 4) == hipSuccess &&
-           hipMalloc(&sl->d_lengths, v * 4) == hipSuccess &&
-           hipMalloc(&sl->d_words, 2 * v * 4) == hipSuccess &&
-           hipMalloc(&sl->d_counts, s->cfg.max_batch_cols * sizeof(wahgpu_counts)) == hipSuccess;
-      sl->counts.resize(s->cfg.max_batch_cols);
+      ok = hipHostMalloc(reinterpret_cast<void**>(&sl->hcounts),
+                         s->cfg.max_batch_cols * sizeof(wahgpu_counts),
+                         hipHostMallocDefault) == hipSuccess;
       for (uint32_t e = 0; ok && e < s->cfg.max_batch_cols; ++e) {
         hipEvent_t ev = nullptr;
         ok = hipEventCreateWithFlags(&ev, hipEventDisableTiming) == hipSuccess;
@@ -261,6 +380,7 @@ int wahgpu_server_create(wahgpu_server** out, const wahgpu_server_config* cfg) {
       (void)hipGetLastError();
       free_slot(sl.get());
       for (auto& o : s->slots) free_slot(o.get());
+      if (s->arena) (void)hipFree(s->arena);
       return rc != WAHGPU_OK ? rc : WAHGPU_ENOMEM;
     }
     s->slots.push_back(std::move(sl));
@@ -278,17 +398,25 @@ void wahgpu_server_destroy(wahgpu_server* s) {
     s->stop = true;
   }
   s->work_cv.notify_all();
+  s->mem_cv.notify_all();
   for (auto& sl : s->slots)
     if (sl->th.joinable()) sl->th.join();
   for (auto& sl : s->slots) free_slot(sl.get());
+  if (s->arena) {
+    (void)hipSetDevice(s->cfg.device);
+    (void)hipFree(s->arena);
+  }
   delete s;
 }
 
 int wahgpu_server_submit(wahgpu_server* s, const uint32_t* values, uint64_t n, uint32_t rid_base,
                          wahgpu_result* result, uint64_t* id) {
   if (!s || !result || !id || (n && !values)) return WAHGPU_EINVAL;
-  if (n > s->slot_values || n > WAHGPU_MAX_VALUES) return WAHGPU_ERANGE;
+  if (n > WAHGPU_MAX_VALUES) return WAHGPU_ERANGE;
   if ((uint64_t)rid_base + n > ((uint64_t)1 << 32)) return WAHGPU_ERANGE;
+  if (!s->cfg.plain && (single_footprint(n) > s->cfg.arena_bytes ||
+                        (s->cfg.max_batch_values && align_values(n) > s->cfg.max_batch_values)))
+    return WAHGPU_ERANGE;  // the empty arena could not hold it
   auto r = std::make_shared<Request>();
   r->values = values;
   r->n = n;
@@ -338,6 +466,14 @@ int wahgpu_server_wait(wahgpu_server* s, uint64_t id) {
   return st;
 }
 
+const char* wahgpu_server_last_error(wahgpu_server* s) {
+  if (!s) return "";
+  std::lock_guard<std::mutex> lk(s->mu);
+  thread_local std::string copy;
+  copy = s->last_error;
+  return copy.c_str();
+}
+
 int wahgpu_server_drain(wahgpu_server* s) {
   if (!s) return WAHGPU_EINVAL;
   std::unique_lock<std::mutex> lk(s->mu);
@@ -355,10 +491,13 @@ int wahgpu_server_get_stats(wahgpu_server* s, wahgpu_server_stats* out) {
     out->batches = s->batches;
     out->values = s->values;
     out->max_batch_cols = s->max_cols;
-    out->slot_values = s->slot_values;
+    out->slot_values = s->cfg.max_batch_values;
     out->busy_s = s->have_first && !lat.empty()
                       ? std::chrono::duration<double>(s->last_done - s->first_submit).count()
                       : 0.0;
+    out->memory_waits = s->memory_waits;
+    out->arena_bytes = s->pool.cap;
+    out->arena_peak = s->pool.peak;
   }
   std::sort(lat.begin(), lat.end());
   auto pct = [&](double q) {
@@ -394,7 +533,8 @@ int wahgpu_server_reset_stats(wahgpu_server* s) {
   if (!s) return WAHGPU_EINVAL;
   std::lock_guard<std::mutex> lk(s->mu);
   s->lat_us.clear();
-  s->batches = s->values = s->max_cols = 0;
+  s->batches = s->values = s->max_cols = s->memory_waits = 0;
+  s->pool.peak = s->pool.used;
   s->have_first = false;
   return WAHGPU_OK;
 }

@@ -14,6 +14,7 @@
 #include <string>
 #include <vector>
 
+#include "wah_internal.h"
 #include "wah_kernels.h"
 #include "wahgpu.h"
 
@@ -752,20 +753,51 @@ uint64_t batch_scratch_bytes(const Caps& shared, size_t nsubs) {
   return build_scratch_bytes(shared, (uint32_t)(nsubs ? nsubs : 1));
 }
 
+// Carving from a caller's buffer instead of the arena, for the lifetime of
+// the guard.
+struct ScratchSwap {
+  wahgpu_ctx* c;
+  char* arena;
+  uint64_t bump;
+  bool on;
+  ScratchSwap(wahgpu_ctx* ctx, char* ext) : c(ctx), arena(ctx->arena), bump(ctx->bump), on(ext) {
+    if (on) {
+      c->arena = ext;
+      c->bump = 0;
+    }
+  }
+  ~ScratchSwap() {
+    if (on) {
+      c->arena = arena;
+      c->bump = bump;
+    }
+  }
+};
+
+// ext: scratch supplied by the caller (ext_bytes long; no arena, no graph).
 int batch_device_impl(wahgpu_ctx* c, const uint32_t* d_values, uint64_t values_len,
                       const wahgpu_column* cols, uint32_t ncols, uint32_t* d_keys,
                       uint32_t* d_offsets, uint32_t* d_lengths, uint32_t* d_words,
-                      wahgpu_counts* d_counts, uint64_t extra_bytes) {
+                      wahgpu_counts* d_counts, uint64_t extra_bytes, char* ext = nullptr,
+                      uint64_t ext_bytes = 0) {
   std::vector<SubBatch> subs;
   Caps shared;
-  const bool reserve = c->res_cols != 0 && c->force_shape < 0;
+  const bool reserve = !ext && c->res_cols != 0 && c->force_shape < 0;
   int rc = plan_batch(c, cols, ncols, values_len, reserve, &subs, &shared);
   if (rc) return rc;
   const bool graph = reserve && reserved_caps(c).covers(shared) && ncols <= c->res_cols;
   const Caps k = graph ? reserved_caps(c) : shared;
-  // extra_bytes: room the caller carves after the scratch (host batches)
-  rc = arena_reserve(c, batch_scratch_bytes(k, subs.size()) + extra_bytes);
-  if (rc) return rc;
+  const uint64_t need = batch_scratch_bytes(k, subs.size());
+  if (ext) {
+    if (need > ext_bytes)
+      return fail(c, WAHGPU_ENOMEM, "workspace of " + std::to_string(ext_bytes) +
+                                        " bytes is too small for " + std::to_string(need));
+  } else {
+    // extra_bytes: room the caller carves after the scratch (host batches)
+    rc = arena_reserve(c, need + extra_bytes);
+    if (rc) return rc;
+  }
+  ScratchSwap swap(c, ext);
   BuildArgs base{};
   carve_build(c, &base, k, 0);
   std::vector<BuildArgs> args;
@@ -1339,6 +1371,37 @@ int wahgpu_segmented_inclusive_scan(wahgpu_ctx* c, const uint32_t* d_in, const u
   wah::launch_scan(s.args, d_in, d_heads, n, d_out, true, c->stream);
   LAUNCHCHK(c);
   return WAHGPU_OK;
+}
+
+// ---- internal: batched build on a caller's workspace (the server) -------
+
+void wah_batch_acc_add(wah_batch_acc* acc, uint64_t n) {
+  uint32_t min_tile = wah::radix_tile(0);
+  for (int sh = 1; sh < wah::kRadixShapes; ++sh) min_tile = std::min(min_tile, wah::radix_tile(sh));
+  acc->values += (n + 63) / 64 * 64;
+  acc->cols += 1;
+  acc->sort_tiles += wah::sort_tiles_of(n, min_tile);
+  acc->tail_tiles += wah::tail_tiles_of(n);
+  acc->hist_chunks += wah::hist_chunks_of(n);
+}
+
+uint64_t wah_batch_acc_scratch(const wah_batch_acc* acc) {
+  // one descriptor region per compiled tile shape: the most sub-batches
+  const Caps k{acc->values, acc->cols, (uint32_t)acc->sort_tiles, (uint32_t)acc->tail_tiles,
+               (uint32_t)acc->hist_chunks};
+  return build_scratch_bytes(k, wah::kRadixShapes);
+}
+
+int wah_build_batch_ws(wahgpu_ctx* c, const uint32_t* d_values, uint64_t values_len,
+                       const wahgpu_column* cols, uint32_t ncols, uint32_t* d_keys,
+                       uint32_t* d_offsets, uint32_t* d_lengths, uint32_t* d_words,
+                       wahgpu_counts* d_counts, void* d_scratch, uint64_t scratch_bytes) {
+  if (!c) return WAHGPU_EINVAL;
+  if (ncols == 0) return WAHGPU_OK;
+  if (!cols || !d_counts || !d_scratch) return fail(c, WAHGPU_EINVAL, "null argument");
+  HIPCHK(c, hipSetDevice(c->device));
+  return batch_device_impl(c, d_values, values_len, cols, ncols, d_keys, d_offsets, d_lengths,
+                           d_words, d_counts, 0, static_cast<char*>(d_scratch), scratch_bytes);
 }
 
 // ---- query side: decode, range query, encode, combine --------------------

@@ -4,6 +4,7 @@ import os
 import subprocess
 import sys
 
+import numpy as np
 import pytest
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
@@ -64,3 +65,93 @@ def test_traffic_profile_parses():
     text = open(bench.TRAFFIC_FILE).read()
     assert "--pmc FETCH_SIZE" in text and "--pmc WRITE_SIZE" in text
     assert "python3 bench.py" in text
+
+
+class _FakeServer:
+    """Stand-in for ia.Server with the same submit / wait / stats surface:
+    a worker thread completes requests in order after a short delay, writing
+    a result that depends only on the column."""
+
+    def __init__(self):
+        import queue
+        import threading
+        self.q = queue.Queue()
+        self.n = 0
+        self.th = threading.Thread(target=self._work, daemon=True)
+        self.th.start()
+
+    def _work(self):
+        import time
+        while True:
+            p = self.q.get()
+            time.sleep(0.0005)
+            k, o, ln, w = p.bufs
+            K = min(3, k.size)
+            k[:K] = p.values[:K]
+            o[:K] = 1
+            ln[:K] = 2
+            w[:2 * K] = p.values[0] if p.values.size else 0
+            p.res.counts.keycnt, p.res.counts.length = K, 2 * K
+            p.done.set()
+
+    def submit(self, values, out):
+        import threading
+        import types
+        p = types.SimpleNamespace(values=values, bufs=out, done=threading.Event(),
+                                  res=types.SimpleNamespace(counts=types.SimpleNamespace(
+                                      keycnt=0, length=0)))
+        self.n += 1
+        self.q.put(p)
+        return p
+
+    def wait(self, p):
+        p.done.wait()
+
+    def reset_stats(self):
+        pass
+
+    def stats(self):
+        return {"p50_us": 0.0, "p99_us": 0.0, "values": 1, "busy_s": 1.0, "requests": self.n,
+                "batches": self.n, "max_batch_cols": 1, "memory_waits": 0, "arena_peak": 0}
+
+
+def test_serve_loop_retires_through_a_full_ring():
+    """The open-loop submitter waits for result-ring space that only the
+    retire thread frees: with a ring that holds a few requests, every request
+    still completes and every result matches its column's recorded one."""
+    import collections
+    import concurrent.futures as cf
+    import threading
+    import zlib
+
+    rng = np.random.default_rng(5)
+    layout, cols = [], []
+    for i in range(16):
+        n = int(rng.integers(1, 400))
+        cols.append(rng.integers(0, 1000, n).astype(np.uint32))
+        layout.append((0, n))
+
+    class Ring(bench._ResultRing):
+        def __init__(self, words):
+            self.buf = np.zeros(words, np.uint32)
+            self.cap = words
+            self.live = collections.deque()
+            self.cv = threading.Condition()
+
+    ring = Ring(5 * 400 + 4 + 100)   # room for about one large request
+    req_col = [i % 16 for i in range(300)]
+    alone = [None] * 16
+
+    def checksum(k, o, ln, w):
+        return zlib.crc32(k.tobytes() + o.tobytes() + ln.tobytes() + w.tobytes())
+
+    with cf.ThreadPoolExecutor(4) as pool:
+        srv = _FakeServer()
+        r1 = bench.serve_run(srv, np.zeros(300), req_col, cols, layout, ring, pool, checksum,
+                             alone, True)
+        assert r1["mismatches"] == 0 and all(a is not None for a in alone)
+        arrivals = np.cumsum(rng.exponential(0.0002, 300))
+        r2 = bench.serve_run(_FakeServer(), arrivals, req_col, cols, layout, ring, pool,
+                             checksum, alone, False)
+        assert r2["mismatches"] == 0
+    assert not ring.live

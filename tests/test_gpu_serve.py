@@ -93,3 +93,52 @@ def test_poll_until_done(ctx):
             assert time.time() - t0 < 60
             time.sleep(0.001)
     assert_same(g, oracle.build(c))
+
+
+def test_request_waits_for_arena_memory_then_runs(ctx):
+    """An arena that holds one 2^22-value request's footprint but not two:
+    the second and third wait until a running batch frees its block, then
+    run; every result is exact and the arena is never overcommitted."""
+    n = 1 << 22
+    cols = [ia.generate(n, "uniform", seed=40 + i, lo=0, hi=(1 << (10 + 3 * i)) - 1)
+            for i in range(3)]
+    arena = 300 << 20
+    with ia.Server(slots=2, arena_bytes=arena, max_batch_cols=8) as srv:
+        print("[serve] three 2^22 requests, arena for one", flush=True)
+        pend = [srv.submit(c, rid_base=i) for i, c in enumerate(cols)]
+        got = [srv.wait(p) for p in pend]
+        st = srv.stats()
+    assert st["memory_waits"] >= 1, st
+    assert st["arena_bytes"] == arena and 0 < st["arena_peak"] <= arena, st
+    assert st["batches"] == 3
+    for i, (c, g) in enumerate(zip(cols, got)):
+        assert_same(g, oracle.build(c, rid_base=i), f"request {i}")
+
+
+def test_request_larger_than_the_arena_is_refused(ctx):
+    with ia.Server(slots=1, arena_bytes=64 << 20) as srv:
+        with pytest.raises(ia.WahError) as e:
+            srv.submit(np.zeros(1 << 22, np.uint32))
+        assert e.value.code == ia._native.ERANGE
+        small = np.arange(100000, dtype=np.uint32) % 77
+        assert_same(srv.wait(srv.submit(small)), oracle.build(small))
+
+
+def test_served_mix_sample_matches_oracle_and_alone(ctx):
+    """A seeded sample of the served mix (columns drawn from the batched
+    workload's distributions, seed 11) against the oracle and against the
+    same column built alone."""
+    from indexing_amd import workloads as W
+    specs = [s for s in W.column_specs(200, 11) if s[0] <= 1 << 21]
+    pick = np.random.default_rng(11).choice(len(specs), 16, replace=False)
+    cols = [W.make_column(specs[i]) for i in pick]
+    with ia.Server(slots=2, arena_bytes=8 << 30, max_batch_cols=1024,
+                   max_batch_values=1 << 24) as srv:
+        print(f"[serve] mix sample of {len(cols)}", flush=True)
+        pend = [srv.submit(c) for c in cols]
+        got = [srv.wait(p) for p in pend]
+    for i, (c, g) in enumerate(zip(cols, got)):
+        assert_same(g, oracle.build(c), f"sample {i}")
+        alone = ctx.build(c)
+        assert ia.index_checksum(g.keys, g.offsets, g.lengths, g.words) == \
+            ia.index_checksum(alone.keys, alone.offsets, alone.lengths, alone.words)

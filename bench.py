@@ -430,7 +430,7 @@ def serve_run(srv, arrivals, req_col, cols, layout, ring, pool, checksum, alone,
             "mismatches": bad, "errors": state.get("errors", [])[:3]}
 
 
-def bench_serve(nreq, pool_cols=512, seed=11, arena_gib=64, slots=2):
+def bench_serve(nreq, pool_cols=512, seed=11, arena_gib=64, slots=3):
     """C5: continuous-batching server against the plain per-column path.
 
     Requests: nreq columns cycling over a pool of pool_cols distinct columns
@@ -575,7 +575,7 @@ def main():
                     help="columns of the batched workload (0 skips it)")
     ap.add_argument("--serve-requests", type=int, default=20000,
                     help="requests of the serving workload (0 skips it)")
-    ap.add_argument("--serve-slots", type=int, default=2, help="server slots (batches in flight)")
+    ap.add_argument("--serve-slots", type=int, default=3, help="server slots (batches in flight)")
     ap.add_argument("--no-ingest", dest="ingest", action="store_false",
                     help="skip the text value-file write / parse timing")
     ap.add_argument("--cpu-baseline-worker", nargs=2, type=int, metavar=("N", "CORE"),

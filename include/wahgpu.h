@@ -79,7 +79,16 @@ int wahgpu_device_compute_units(int ordinal);
 int wahgpu_find_device(const char* name);
 /* One HIP stream plus a pooled device arena.  arena_bytes = 0 grows the arena
  * on demand; otherwise the arena is allocated once and requests beyond it
- * fail with WAHGPU_ENOMEM.  A context is used by one thread at a time. */
+ * fail with WAHGPU_ENOMEM.  A context is used by one thread at a time.
+ * Contexts and streams may run builds at the same time, but on one device the
+ * library's single-pass kernels (radix passes, tail, scans, compaction,
+ * decode) run one call's launch sequence at a time, in the order the calls
+ * were made: each call's kernels wait (hipStreamWaitEvent) for the previous
+ * call's, whatever its stream.  Their look-back waits on tiles drawn by
+ * workgroups placed on particular XCDs, and two such launches sharing the
+ * device could each hold the CUs the other needs.  Copies and other work still
+ * overlap them.  Calls made while their stream captures a graph are not
+ * ordered: replay such graphs on one stream at a time. */
 int wahgpu_create(wahgpu_ctx** out, int device_ordinal, uint64_t arena_bytes);
 void wahgpu_destroy(wahgpu_ctx* ctx);
 const char* wahgpu_last_error(const wahgpu_ctx* ctx);
@@ -120,8 +129,9 @@ int wahgpu_build_device(wahgpu_ctx* ctx, const uint32_t* d_values, uint64_t n,
  * and outputs are made beforehand.  Columns of at most
  * WAHGPU_NOALLOC_MAX_VALUES values; larger ones fail with WAHGPU_ERANGE.
  * Every call on one GPU shares that GPU's array (the build clears what it
- * needs itself): calls that overlap on one GPU, from other streams or
- * contexts, race and must be ordered by the caller. */
+ * needs itself).  Calls from other streams or contexts are ordered like every
+ * build (see wahgpu_create); calls captured into graphs are not, and graphs
+ * replayed on several streams at once race on the array. */
 #define WAHGPU_NOALLOC_MAX_VALUES (((uint64_t)1) << 22)
 int wahgpu_build_device_noalloc(wahgpu_ctx* ctx, const uint32_t* d_values, uint64_t n,
                                 uint32_t rid_base, uint32_t* d_keys, uint32_t* d_offsets,
@@ -245,8 +255,9 @@ int wahgpu_set_tuning(wahgpu_ctx* ctx, const wahgpu_tuning* in);
  * When the first queued request does not fit the free memory, it waits until a
  * running batch ends and frees its block (admission by HBM footprint).
  * Requests arriving while others are on the GPU join the next batch of the
- * first slot to free up, and the copies and kernels of different slots
- * overlap on the device.  A request completes as soon as its own result
+ * first slot to free up; the copies of different slots overlap each other
+ * and the kernels of the others (the kernels themselves run one batch at a
+ * time, as every build on a device).  A request completes as soon as its own result
  * copies have ended, in batch order; its block is freed after the batch's last
  * copy.  With plain = 1 every slot instead serves one request at a time
  * through wahgpu_build (the per-column drop-in, with its own growing arena),

@@ -190,8 +190,9 @@ class Context:
         """build_device without any device allocation by the library: the
         scratch is a static 112 MiB device array per GPU
         (wahgpu_build_device_noalloc; columns of at most NOALLOC_MAX_VALUES).
-        out, if given, is reused as in build_device; calls on one GPU that
-        overlap race."""
+        out, if given, is reused as in build_device.  Calls from other
+        streams are ordered by the library; graphs that captured this call
+        and are replayed on several streams at once race."""
         torch = _torch()
         n = int(d_values.numel())
         if out is None:

@@ -67,9 +67,14 @@ __device__ __forceinline__ uint64_t shfl64(uint64_t x, int src) {
 // draws from counter b % lanes and gets tiles lane, lane + lanes, ... in
 // increasing order.  A global atomic on one address costs ~11 ns on MI355X
 // and they serialize, so spreading the draws over lanes removes that floor.
-// Every tile below a drawn tile was drawn earlier by a running block of its
-// lane, so look-back waits always end.  WAH_STATIC_TILES=1 maps
-// tile = blockIdx.x instead (one tile per block), relying on in-order dispatch.
+// A lane's draws ascend, so the lowest unfinished tile is either held by a
+// running block or is the next draw of its lane; grids are sized so that all
+// their blocks fit on the device at once, so a block of that lane draws it
+// and look-back waits end.  That needs every lane to get its blocks placed:
+// a lane whose XCD is held by another launch that is waiting itself would
+// stall the rest, so the host never overlaps two launch sequences of these
+// kernels on one device.  WAH_STATIC_TILES=1 maps tile = blockIdx.x instead
+// (one tile per block), relying on in-order dispatch.
 #ifndef WAH_STATIC_TILES
 #define WAH_STATIC_TILES 0
 #endif
@@ -81,8 +86,7 @@ constexpr uint32_t kLaneStride = 64;
 // Workgroups are dispatched to the 8 XCDs round robin, so lane = block % 8
 // is the XCD, and neighbouring tiles then write the cache lines they share
 // (the ends of their output runs) through one L2, which merges them.  A
-// lane's draws still ascend, so every tile below a drawn tile is drawn by a
-// running block of its lane, and look-back waits end.
+// lane's draws still ascend, so the argument above holds.
 __device__ __forceinline__ uint32_t next_tile(uint32_t* counters, uint32_t lanes,
                                               uint32_t* s_tile, uint32_t& iter,
                                               uint32_t group = 1) {

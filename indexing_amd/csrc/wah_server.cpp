@@ -12,8 +12,9 @@
 // builds them as one batch in that block; when even the first queued request
 // does not fit, the slot waits until a running batch ends and frees its block.
 // While slots copy or build, arriving requests queue, so batches grow with
-// the load (continuous batching), and the transfers and kernels of different
-// slots overlap on the device.
+// the load (continuous batching); the transfers of a slot overlap the other
+// slots' transfers and kernels (the library runs the kernels of one build
+// at a time per device).
 #include <hip/hip_runtime_api.h>
 #include <stdint.h>
 

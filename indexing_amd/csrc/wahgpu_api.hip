@@ -76,6 +76,62 @@ int fail(wahgpu_ctx* c, int code, const std::string& msg) {
       return fail(ctx, WAHGPU_EHIP, std::string("kernel launch: ") + hipGetErrorString(e_)); \
   } while (0)
 
+// ---- one look-back launch sequence at a time per device -----------------
+// The single-pass kernels (radix passes, tail, merge, compaction, scans,
+// decode) wait for tiles that other workgroups of the same launch draw, and
+// a tile-counter lane is drained only by workgroups placed on its XCD.  Two
+// such launches on different streams can each fill the XCD the other one
+// still needs while waiting on their own undrained lane: a circular wait that
+// only the bounded wait ends, by reporting a fault.  So on one device these
+// kernels never overlap.  Each launch sequence, from any context and on any
+// stream, waits for a per-device event that the previous sequence recorded
+// and records it after itself; the host lock keeps wait, launches and record
+// of one sequence together.  Copies and other kernels still overlap them.
+// Sequences enqueued while their stream is being captured into a graph are
+// not ordered: a caller replays such graphs on one stream at a time.
+constexpr int kOrderDevices = 64;
+struct OrderSlot {
+  std::recursive_mutex mu;
+  hipEvent_t last = nullptr;  // recorded after the latest sequence
+};
+OrderSlot g_order[kOrderDevices];
+
+class DeviceOrder {
+ public:
+  explicit DeviceOrder(wahgpu_ctx* c) : c_(c) {
+    if (c->device < 0 || c->device >= kOrderDevices) {
+      st_ = hipErrorInvalidDevice;
+      return;
+    }
+    slot_ = &g_order[c->device];
+    slot_->mu.lock();
+    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+    st_ = hipStreamIsCapturing(c->stream, &cs);
+    if (st_ != hipSuccess || cs != hipStreamCaptureStatusNone) return;
+    if (!slot_->last) st_ = hipEventCreateWithFlags(&slot_->last, hipEventDisableTiming);
+    if (st_ == hipSuccess) st_ = hipStreamWaitEvent(c->stream, slot_->last, 0);
+    record_ = st_ == hipSuccess;
+  }
+  ~DeviceOrder() {
+    if (!slot_) return;
+    if (record_) (void)hipEventRecord(slot_->last, c_->stream);
+    slot_->mu.unlock();
+  }
+  DeviceOrder(const DeviceOrder&) = delete;
+  DeviceOrder& operator=(const DeviceOrder&) = delete;
+  hipError_t status() const { return st_; }
+
+ private:
+  wahgpu_ctx* c_;
+  OrderSlot* slot_ = nullptr;
+  hipError_t st_ = hipSuccess;
+  bool record_ = false;
+};
+
+#define ORDERED(ctx)                  \
+  DeviceOrder order_guard_(ctx);      \
+  HIPCHK(ctx, order_guard_.status())
+
 constexpr uint64_t kAlign = 256;
 uint64_t align_up(uint64_t x) { return (x + kAlign - 1) / kAlign * kAlign; }
 
@@ -510,6 +566,7 @@ static int build_device_impl(wahgpu_ctx* c, const uint32_t* d_values, uint64_t n
     ++c->tcalls;
     hook.mark(c->stream);
   }
+  ORDERED(c);
   wah::launch_sort(a, c->stream, hp);
   LAUNCHCHK(c);
   if (marks) marks->mark(c->stream);
@@ -816,6 +873,7 @@ int batch_device_impl(wahgpu_ctx* c, const uint32_t* d_values, uint64_t values_l
     if (rc) return rc;
     args.push_back(a);
   }
+  ORDERED(c);
   if (graph) {
     std::vector<uintptr_t> key = {(uintptr_t)c->arena, (uintptr_t)c->stream,
                                   (uintptr_t)d_values, (uintptr_t)d_keys,
@@ -1149,6 +1207,7 @@ int wahgpu_build_staged(wahgpu_ctx* c, const uint32_t* values, uint64_t n, uint3
   a.force_pair = true;  // the stages below take one (value, rid) pair per row
   // The passes ping-pong between (keysA, ridsA) and (keysB, ridsB); finalize
   // copies whichever holds the result into (s_keys, s_rids).
+  ORDERED(c);
   wah::launch_sort(a, c->stream);
   wah::launch_sort_finalize(a, c->stream);
   LAUNCHCHK(c);
@@ -1235,6 +1294,7 @@ int wahgpu_sort_rids_by_value(wahgpu_ctx* c, const uint32_t* d_values, uint64_t 
   a.sorted_keys = d_keys_out;
   a.sorted_rids = d_rids_out;
   a.force_pair = true;
+  ORDERED(c);
   wah::launch_sort(a, c->stream);
   wah::launch_sort_finalize(a, c->stream);
   LAUNCHCHK(c);
@@ -1264,6 +1324,7 @@ int wahgpu_merge_lit_by_val_chids(wahgpu_ctx* c, const uint32_t* d_values,
   rc = arena_reserve(c, scan_scratch_bytes(n));
   if (rc) return rc;
   ScanScratch s = carve_scan(c, n);
+  ORDERED(c);
   wah::launch_fill32(s.base, s.words, 0, c->stream);
   wah::launch_merge(s.args, d_values, d_chids, d_lits, n, d_values_out, d_chids_out, d_lits_out,
                     s.result, c->stream);
@@ -1291,6 +1352,7 @@ int wahgpu_fuse_fill_literals(wahgpu_ctx* c, const uint32_t* d_fills, const uint
   rc = arena_reserve(c, scan_scratch_bytes(k));
   if (rc) return rc;
   ScanScratch s = carve_scan(c, k);
+  ORDERED(c);
   wah::launch_fill32(s.base, s.words, 0, c->stream);
   wah::launch_compact(s.args, d_fills, d_lits, k, 1, d_index, s.result, c->stream);
   LAUNCHCHK(c);
@@ -1313,6 +1375,7 @@ int wahgpu_compute_column_length(wahgpu_ctx* c, const uint32_t* d_values,
   wah::launch_fill32(s.base, s.words, 0, c->stream);
   wah::launch_fill32(reinterpret_cast<uint32_t*>(d_counts), sizeof(wahgpu_counts) / 4, 0,
                      c->stream);
+  ORDERED(c);
   wah::launch_colen(s.args, d_values, d_fills, k, d_keys, d_offsets, d_lengths, d_counts,
                     c->stream);
   LAUNCHCHK(c);
@@ -1336,6 +1399,7 @@ int wahgpu_stream_compaction(wahgpu_ctx* c, const uint32_t* d_data, const uint32
   rc = arena_reserve(c, scan_scratch_bytes(n));
   if (rc) return rc;
   ScanScratch s = carve_scan(c, n);
+  ORDERED(c);
   wah::launch_fill32(s.base, s.words, 0, c->stream);
   wah::launch_compact(s.args, d_data, d_valid, n, 0, d_out, s.result, c->stream);
   LAUNCHCHK(c);
@@ -1351,6 +1415,7 @@ int wahgpu_exclusive_scan(wahgpu_ctx* c, const uint32_t* d_in, uint64_t n, uint3
   rc = arena_reserve(c, scan_scratch_bytes(n));
   if (rc) return rc;
   ScanScratch s = carve_scan(c, n);
+  ORDERED(c);
   wah::launch_fill32(s.base, s.words, 0, c->stream);
   wah::launch_scan(s.args, d_in, nullptr, n, d_out, false, c->stream);
   LAUNCHCHK(c);
@@ -1367,6 +1432,7 @@ int wahgpu_segmented_inclusive_scan(wahgpu_ctx* c, const uint32_t* d_in, const u
   rc = arena_reserve(c, scan_scratch_bytes(n));
   if (rc) return rc;
   ScanScratch s = carve_scan(c, n);
+  ORDERED(c);
   wah::launch_fill32(s.base, s.words, 0, c->stream);
   wah::launch_scan(s.args, d_in, d_heads, n, d_out, true, c->stream);
   LAUNCHCHK(c);
@@ -1422,6 +1488,7 @@ int wahgpu_decode_device(wahgpu_ctx* c, const uint32_t* d_words, uint64_t len, u
   if (rc) return rc;
   ScanScratch sc = carve_scan(c, len);
   uint64_t* d_last = take<uint64_t>(c, 2);
+  ORDERED(c);
   wah::launch_fill32(sc.base, sc.words, 0, c->stream);
   wah::launch_fill32(reinterpret_cast<uint32_t*>(d_last), 4, 0, c->stream);
   if (nrows) wah::launch_fill32(d_bitmap, bitmap_words(nrows), 0, c->stream);
@@ -1468,6 +1535,7 @@ int wahgpu_query_range_device(wahgpu_ctx* c, const uint32_t* d_keys, const uint3
   if (rc) return rc;
   ScanScratch sc = carve_scan(c, len);
   uint32_t* heads = take<uint32_t>(c, len);
+  ORDERED(c);
   wah::launch_fill32(sc.base, sc.words, 0, c->stream);
   wah::launch_fill32(heads, len, 0, c->stream);
   wah::launch_mark_heads(d_offsets, range[0], range[1], w0, heads, c->stream);
@@ -1521,6 +1589,7 @@ int wahgpu_combine_device(wahgpu_ctx* c, int op, const uint32_t* d_a, uint64_t l
   ScanScratch sa = carve_scan(c, la), sb = carve_scan(c, lb);
   uint64_t* tile_sum = take<uint64_t>(c, 3 * tiles);
   uint64_t* d_len = take<uint64_t>(c, 1);
+  ORDERED(c);
   wah::launch_fill32(sa.base, sa.words, 0, c->stream);
   wah::launch_fill32(sb.base, sb.words, 0, c->stream);
   wah::launch_fill32(bm_a, bw, 0, c->stream);

@@ -222,20 +222,24 @@ int wahgpu_graph_stats(const wahgpu_ctx* ctx, uint64_t* captures, uint64_t* repl
  * radix tile shape of its length bucket (columns of different shapes in one
  * batch run as one launch set per shape):
  *   buckets:  n < 2^14, n < 2^17, n < 2^20, n < 2^24, larger;
- *   shapes:   0 = 1024 threads x 16 keys, 1 = 512 x 16, 2 = 256 x 16
- *             (8-bit digits in every shape; the tail tile is 1024 x 8).
+ *   shapes:   threads per block x keys per thread of the radix passes,
+ *             0 = 1024 x 16, 1 = 512 x 16, 2 = 256 x 16, 3 = 1024 x 8,
+ *             4 = 512 x 8, 5 = 512 x 24 (shape_threads / shape_items; every
+ *             tile a multiple of 4096 keys; the tail tile is 1024 x 8).
  * wahgpu_autotune times every shape on synthetic batches of each bucket
  * (uniform values, 2^12 distinct; columns of 2^13, 2^15.5, 2^18.5, 2^22 and
  * 2^26 values, 2^24 values per bucket and one 2^26 column for the last) on
  * this device, keeps the fastest per bucket in the context and reports it
  * with the timings.  The default table is shape 0 everywhere. */
 #define WAHGPU_TUNE_BUCKETS 5
-#define WAHGPU_TUNE_SHAPES 3
+#define WAHGPU_TUNE_SHAPES 6
 typedef struct wahgpu_tuning {
   uint32_t bucket_log2[WAHGPU_TUNE_BUCKETS]; /* bucket b: n < 2^bucket_log2[b] (64: any) */
   uint32_t shape[WAHGPU_TUNE_BUCKETS];       /* chosen shape per bucket */
   uint32_t digit_bits[WAHGPU_TUNE_BUCKETS];  /* radix digit width per bucket (8) */
   double ms[WAHGPU_TUNE_BUCKETS][WAHGPU_TUNE_SHAPES]; /* autotune: build time per shape */
+  uint32_t shape_threads[WAHGPU_TUNE_SHAPES];  /* threads per block of each shape */
+  uint32_t shape_items[WAHGPU_TUNE_SHAPES];    /* keys per thread of each shape */
 } wahgpu_tuning;
 int wahgpu_autotune(wahgpu_ctx* ctx, wahgpu_tuning* out);
 int wahgpu_get_tuning(const wahgpu_ctx* ctx, wahgpu_tuning* out);

@@ -298,7 +298,8 @@ class Context:
         return self._tuning_dict(t)
 
     def set_tuning(self, shapes):
-        """shapes: radix tile shape per bucket (0: 1024x16, 1: 512x16, 2: 256x16)."""
+        """shapes: radix tile shape per bucket, an index into
+        tuning()["shapes"] ((threads, keys per thread) of each shape)."""
         t = N.Tuning()
         for b, sh in enumerate(shapes):
             t.shape[b] = int(sh)
@@ -308,7 +309,8 @@ class Context:
     def _tuning_dict(t):
         return {"bucket_log2": list(t.bucket_log2), "shape": list(t.shape),
                 "digit_bits": list(t.digit_bits),
-                "ms": [[round(x, 4) for x in row] for row in t.ms]}
+                "ms": [[round(x, 4) for x in row] for row in t.ms],
+                "shapes": [(int(a), int(b)) for a, b in zip(t.shape_threads, t.shape_items)]}
 
     def graph_stats(self):
         cap, rep = ctypes.c_uint64(0), ctypes.c_uint64(0)

@@ -72,14 +72,16 @@ class ServerStats(ctypes.Structure):
                 ("arena_peak", ctypes.c_uint64)]
 
 
-TUNE_BUCKETS, TUNE_SHAPES = 5, 3
+TUNE_BUCKETS, TUNE_SHAPES = 5, 6
 
 
 class Tuning(ctypes.Structure):
     _fields_ = [("bucket_log2", ctypes.c_uint32 * TUNE_BUCKETS),
                 ("shape", ctypes.c_uint32 * TUNE_BUCKETS),
                 ("digit_bits", ctypes.c_uint32 * TUNE_BUCKETS),
-                ("ms", (ctypes.c_double * TUNE_SHAPES) * TUNE_BUCKETS)]
+                ("ms", (ctypes.c_double * TUNE_SHAPES) * TUNE_BUCKETS),
+                ("shape_threads", ctypes.c_uint32 * TUNE_SHAPES),
+                ("shape_items", ctypes.c_uint32 * TUNE_SHAPES)]
 
 
 # name -> (restype, argtypes); every function the header declares.

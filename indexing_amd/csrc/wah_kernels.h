@@ -39,12 +39,31 @@ namespace wah {
 constexpr int kSortThreads = WAH_SORT_THREADS;  // threads of a radix-pass block
 constexpr int kSortItems = WAH_SORT_ITEMS;      // keys per thread in a radix pass
 constexpr int kSortTile = kSortThreads * kSortItems;
-// Radix tile shapes compiled in (shape 0 is the default); a build runs one
-// shape, chosen per column-length bucket by the tuning table.
-constexpr int kRadixShapes = 3;
-constexpr int kRadixThreads[kRadixShapes] = {kSortThreads, 512, 256};
-__host__ __device__ inline uint32_t radix_tile(int shape) {
-  return shape == 1 ? 512u * kSortItems : shape == 2 ? 256u * kSortItems : (uint32_t)kSortTile;
+// Radix tile shapes compiled in: threads per block x keys per thread (shape
+// 0 is the default).  A build runs one shape per column, chosen per
+// column-length bucket by the tuning table.  Fewer keys per thread make more,
+// smaller tiles (more blocks in flight, more look-back steps); more keys per
+// thread the reverse.  Every tile is a whole number of the histogram's
+// 4096-row chunks (and of the run cut's 4096-row steps), which count and cut
+// the runs of one radix tile at a time: so no 256 x 8 or 256 x 24, and no
+// 1024 x 24 (its 96 KiB of staged keys and 16 waves leave too few registers).
+constexpr int kRadixShapes = 6;
+__host__ __device__ constexpr int radix_threads(int shape) {
+  return shape == 1 || shape == 4 || shape == 5 ? 512
+         : shape == 2                           ? 256
+         : shape == 3                           ? 1024
+                                                : kSortThreads;
+}
+__host__ __device__ constexpr int radix_items(int shape) {
+  return shape == 3 || shape == 4 ? 8 : shape == 5 ? 24 : shape == 0 ? kSortItems : 16;
+}
+__host__ __device__ constexpr uint32_t radix_tile(int shape) {
+  return (uint32_t)(radix_threads(shape) * radix_items(shape));
+}
+// Waves per SIMD a radix instantiation is compiled for: 8 keys per thread
+// fit in 80 registers (6 waves), 16 in 128 (4), 24 in 256 (2).
+__host__ __device__ constexpr int sort_waves_per_eu(int items) {
+  return items <= 8 ? 6 : items >= 24 ? 2 : WAH_SORT_WAVES_PER_EU;
 }
 constexpr int kTailThreads = WAH_TAIL_THREADS;
 constexpr int kTailItems = WAH_TAIL_ITEMS;
@@ -95,6 +114,10 @@ constexpr uint32_t kBufB = 2;
 
 constexpr uint32_t kLaneWords = 8 * 64;     // tile-counter lanes of one kernel
 constexpr uint32_t kHistChunk = 4096;       // values per histogram work item
+constexpr bool radix_tiles_whole_chunks(int sh = 0) {
+  return sh == kRadixShapes || (radix_tile(sh) % kHistChunk == 0 && radix_tiles_whole_chunks(sh + 1));
+}
+static_assert(radix_tiles_whole_chunks(), "radix tiles hold whole histogram chunks");
 
 // One column of a build.  Its values are values[off, off + n) of the
 // concatenated input; its sorted pairs, keys, offsets and lengths use the same
@@ -179,7 +202,7 @@ struct BuildArgs {
   uint32_t *sorted_keys, *sorted_rids;  // stage API sort output (one column)
   wahgpu_counts_dev* counts;         // per column
   uint32_t max_resident;             // resident 256-thread blocks on the device
-  int shape;                         // radix tile shape (kRadixThreads[shape] x kSortItems)
+  int shape;                         // radix tile shape (radix_threads x radix_items)
   // Upper bounds the grids are sized for (a captured graph replays any batch
   // within them; the kernels read the actual sizes from the descriptor).
   uint32_t cols_cap, sort_tiles_cap, tail_tiles_cap, hist_chunks_cap;

@@ -594,7 +594,7 @@ __global__ __launch_bounds__(256) void k_rank_selftest(uint32_t* ok) {
 // literal bits) items the run cut wrote to B: launch 0 reads B.
 // ---------------------------------------------------------------------------
 template <int THREADS, int ITEMS, bool RUN, int RANK>
-__global__ __launch_bounds__(THREADS, WAH_SORT_WAVES_PER_EU) void k_onesweep(
+__global__ __launch_bounds__(THREADS, sort_waves_per_eu(ITEMS)) void k_onesweep(
     uint32_t a, const uint32_t* __restrict__ values, uint32_t* keysA, uint32_t* ridsA,
     uint32_t* keysB, uint32_t* ridsB, uint32_t* litsA, uint32_t* litsB,
     const DescHead* __restrict__ desc, const ColState* __restrict__ cs, Header* __restrict__ hdr,
@@ -1619,11 +1619,11 @@ static uint32_t cap_grid(uint32_t work, uint32_t cap) {
 }
 
 // Radix blocks resident per CU: the waves the register budget allows
-// (WAH_SORT_WAVES_PER_EU per SIMD) and the LDS of k_onesweep (staged keys,
+// (sort_waves_per_eu per SIMD) and the LDS of k_onesweep (staged keys,
 // per-wave digit counts, digit tables; 160 KiB per CU).
 template <int T, int I>
 constexpr uint32_t sort_blocks_per_cu() {
-  constexpr uint32_t by_waves = WAH_SORT_WAVES_PER_EU * 4 / (T / kWave);
+  constexpr uint32_t by_waves = sort_waves_per_eu(I) * 4 / (T / kWave);
   constexpr uint32_t lds = T * I * 4 + (T / kWave) * 1024 + 2 * 1024 + 64;
   constexpr uint32_t by_lds = 160u * 1024u / lds;
   constexpr uint32_t b = by_waves < by_lds ? by_waves : by_lds;
@@ -1674,10 +1674,17 @@ void launch_sort(const BuildArgs& a, hipStream_t s, LaunchHook* hook) {
   hipLaunchKernelGGL(k_plan, dim3(cap_grid(a.cols_cap, 1024)), dim3(kBlock), 0, s, a.desc, a.cs,
                      a.hdr, a.tile_runs, a.counts, (int)a.force_pair, radix_tile(a.shape));
   switch (a.shape) {
-    case 1: launch_radix<512, kSortItems>(a, s, hook); break;
-    case 2: launch_radix<256, kSortItems>(a, s, hook); break;
-    default: launch_radix<kSortThreads, kSortItems>(a, s, hook); break;
+#define WAH_SHAPE_CASE(sh) \
+  case sh: launch_radix<radix_threads(sh), radix_items(sh)>(a, s, hook); break;
+    WAH_SHAPE_CASE(1)
+    WAH_SHAPE_CASE(2)
+    WAH_SHAPE_CASE(3)
+    WAH_SHAPE_CASE(4)
+    WAH_SHAPE_CASE(5)
+#undef WAH_SHAPE_CASE
+    default: launch_radix<radix_threads(0), radix_items(0)>(a, s, hook); break;
   }
+  static_assert(kRadixShapes == 6, "a case per compiled shape");
 }
 
 void launch_tail(const BuildArgs& a, hipStream_t s, LaunchHook* hook) {

@@ -1042,6 +1042,10 @@ int wahgpu_get_tuning(const wahgpu_ctx* c, wahgpu_tuning* out) {
     out->digit_bits[b] = 8;
     for (uint32_t s = 0; s < WAHGPU_TUNE_SHAPES; ++s) out->ms[b][s] = c->tune_ms[b][s];
   }
+  for (int sh = 0; sh < WAHGPU_TUNE_SHAPES; ++sh) {
+    out->shape_threads[sh] = (uint32_t)wah::radix_threads(sh);
+    out->shape_items[sh] = (uint32_t)wah::radix_items(sh);
+  }
   return WAHGPU_OK;
 }
 
@@ -1053,6 +1057,8 @@ static void drop_graph(wahgpu_ctx* c) {
     c->gkey.clear();
   }
 }
+
+static_assert(WAHGPU_TUNE_SHAPES == wah::kRadixShapes, "public shape count");
 
 int wahgpu_set_tuning(wahgpu_ctx* c, const wahgpu_tuning* in) {
   if (!c || !in) return WAHGPU_EINVAL;

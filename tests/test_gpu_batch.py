@@ -149,13 +149,19 @@ def test_graph_replays_match_eager(ctx):
         g.close()
 
 
-@pytest.mark.parametrize("shape", [0, 1, 2])
+SHAPES = [(1024, 16), (512, 16), (256, 16), (1024, 8), (512, 8), (512, 24)]
+
+
+@pytest.mark.parametrize("shape", range(N.TUNE_SHAPES))
 def test_every_tile_shape_matches_oracle(shape):
-    """Each compiled radix tile shape, for single columns and a batch."""
+    """Each compiled radix tile shape (threads x keys per thread), for single
+    columns (pair mode, 32-bit keys, run mode) and a batch."""
     c = ia.Context(0)
     try:
         c.set_tuning([shape] * N.TUNE_BUCKETS)
-        assert c.tuning()["shape"] == [shape] * N.TUNE_BUCKETS
+        t = c.tuning()
+        assert t["shape"] == [shape] * N.TUNE_BUCKETS
+        assert t["shapes"] == SHAPES
         rng = np.random.default_rng(40 + shape)
         singles = [rng.integers(0, 65536, 100003).astype(np.uint32),
                    rng.integers(0, 1 << 32, 9000, dtype=np.uint64).astype(np.uint32),
@@ -181,7 +187,7 @@ def test_mixed_shapes_in_one_batch_and_graph():
     import torch
     c = ia.Context(0)
     try:
-        c.set_tuning([2, 1, 0, 1, 2])
+        c.set_tuning([5, 4, 3, 2, 1])
         cols = mixed_columns(71, 40, max_log=22)
         buf, layout = pack(cols)
         ref = [oracle.build(col) for col in cols]

@@ -266,6 +266,8 @@ def bench_batched(ncols, seed=7, reps=3):
             "per_column_ms": round(best_p, 3),
             "per_column_gvalues_s": round(total / best_p / 1e6, 2),
             "speedup": round(best_p / best_b, 2), "tile_shapes": tuning["shape"],
+            "digit_bits": tuning["digit_bits"],
+            "wide_digit_columns": sum(int(c.mode) == 2 for c in counts),
             "graph_captures": captures,
             "graph_replays": replays, "host_generation_s": round(gen_s, 1)}
 
@@ -695,7 +697,7 @@ def main():
         "config": {"workload": f"single-column WAH build, n=2^{n.bit_length() - 1} uint32 "
                                f"{column}, resident in HBM",
                    "n": n, "k": int(counts.k), "L": L, "K": K, "radix_passes": P,
-                   "sort_mode": "runs" if counts.mode else "pairs",
+                   "sort_mode": {0: "pairs", 1: "runs", 2: "pairs, wide digits"}[int(counts.mode)],
                    "radix_ranking": "lds-fetch-add" if rank_mode else "ballot-match",
                    "tuning": tuning},
         "roofline": {"bound": "hbm", "kernel": "k_onesweep (radix pass, mean of active launches)",

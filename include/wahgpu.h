@@ -59,7 +59,8 @@ typedef struct wahgpu_counts {
   uint32_t mode;    /* sort mode of the fused build: 0 = one (value, rid)
                        pair per row, 1 = one item per run of equal values
                        inside a 31-row chunk (chosen when that sorts under
-                       2/3 of the rows); staged paths report 0           */
+                       2/3 of the rows), 2 = pairs sorted with wide digits
+                       (see wahgpu_tuning); staged paths report 0        */
 } wahgpu_counts;
 
 /* Microseconds per phase, the buckets `phases` prints (src/phases.cpp:714-1127). */
@@ -218,32 +219,43 @@ int wahgpu_batch_reserve(wahgpu_ctx* ctx, uint64_t max_values, uint32_t max_cols
 /* Number of graph captures and replays so far (for tests and tuning). */
 int wahgpu_graph_stats(const wahgpu_ctx* ctx, uint64_t* captures, uint64_t* replays);
 
-/* Tile shapes per column-length bucket.  A build sorts each column with the
- * radix tile shape of its length bucket (columns of different shapes in one
- * batch run as one launch set per shape):
+/* Tuning per column-length bucket: the radix tile shape and the digit width.
+ * A build sorts each column with the tile shape of its length bucket
+ * (columns of different shapes in one batch run as one launch set per shape):
  *   buckets:  n < 2^14, n < 2^17, n < 2^20, n < 2^24, larger;
  *   shapes:   threads per block x keys per thread of the radix passes,
  *             0 = 1024 x 16, 1 = 512 x 16, 2 = 256 x 16, 3 = 1024 x 8,
  *             4 = 512 x 8, 5 = 512 x 24 (shape_threads / shape_items; every
- *             tile a multiple of 4096 keys; the tail tile is 1024 x 8).
- * wahgpu_autotune times every shape on synthetic batches of each bucket
- * (uniform values, 2^12 distinct; columns of 2^13, 2^15.5, 2^18.5, 2^22 and
- * 2^26 values, 2^24 values per bucket and one 2^26 column for the last) on
- * this device, keeps the fastest per bucket in the context and reports it
- * with the timings.  The default table is shape 0 everywhere. */
+ *             tile a multiple of 4096 keys; the tail tile is 1024 x 8);
+ *   digits:   8 = 8-bit digits (256 bins; digits every value shares are
+ *             skipped), 11 = wide digits allowed: a pair-mode column of at
+ *             least 2^14 values whose keys take fewer passes of up to 11 bits
+ *             (keys of 9-11, 17-22 or 25-32 significant bits) sorts with
+ *             those (a second histogram pass over its values, then one pass
+ *             per wide digit, 512 x 16 tiles).  counts.mode reports 2 for it
+ *             and counts.passes its wide passes.
+ * wahgpu_autotune times, on this device, synthetic batches of each bucket
+ * (columns of 2^13, 2^15.5, 2^18.5, 2^22 and 2^26 values, 2^24 values per
+ * bucket and one 2^26 column for the last): every shape with 8-bit digits on
+ * uniform values with 2^12 distinct (ms), then the fastest shape with 8-bit
+ * and with wide digits on uniform values with 2^10 distinct (digit_ms), and
+ * keeps the fastest of each per bucket in the context.  The default table is
+ * shape 0 and 8-bit digits everywhere. */
 #define WAHGPU_TUNE_BUCKETS 5
 #define WAHGPU_TUNE_SHAPES 6
 typedef struct wahgpu_tuning {
   uint32_t bucket_log2[WAHGPU_TUNE_BUCKETS]; /* bucket b: n < 2^bucket_log2[b] (64: any) */
   uint32_t shape[WAHGPU_TUNE_BUCKETS];       /* chosen shape per bucket */
-  uint32_t digit_bits[WAHGPU_TUNE_BUCKETS];  /* radix digit width per bucket (8) */
+  uint32_t digit_bits[WAHGPU_TUNE_BUCKETS];  /* digit width per bucket: 8, or 11 (wide allowed) */
   double ms[WAHGPU_TUNE_BUCKETS][WAHGPU_TUNE_SHAPES]; /* autotune: build time per shape */
   uint32_t shape_threads[WAHGPU_TUNE_SHAPES];  /* threads per block of each shape */
   uint32_t shape_items[WAHGPU_TUNE_SHAPES];    /* keys per thread of each shape */
+  double digit_ms[WAHGPU_TUNE_BUCKETS][2];     /* autotune: 8-bit / wide digits (0: not timed) */
 } wahgpu_tuning;
 int wahgpu_autotune(wahgpu_ctx* ctx, wahgpu_tuning* out);
 int wahgpu_get_tuning(const wahgpu_ctx* ctx, wahgpu_tuning* out);
-/* Installs shape[] (other fields ignored); shapes must be < WAHGPU_TUNE_SHAPES. */
+/* Installs shape[] and digit_bits[] (0 reads as 8; other fields ignored);
+ * shapes must be < WAHGPU_TUNE_SHAPES and digit widths 8 or 11. */
 int wahgpu_set_tuning(wahgpu_ctx* ctx, const wahgpu_tuning* in);
 
 /* ---- serving: continuous batching of build requests ----------------------

@@ -297,12 +297,15 @@ class Context:
         self._check(self._lib.wahgpu_get_tuning(self._h, ctypes.byref(t)))
         return self._tuning_dict(t)
 
-    def set_tuning(self, shapes):
+    def set_tuning(self, shapes, digit_bits=None):
         """shapes: radix tile shape per bucket, an index into
-        tuning()["shapes"] ((threads, keys per thread) of each shape)."""
+        tuning()["shapes"] ((threads, keys per thread) of each shape);
+        digit_bits: 8 or 11 (wide digits allowed) per bucket, default 8."""
         t = N.Tuning()
         for b, sh in enumerate(shapes):
             t.shape[b] = int(sh)
+        for b, d in enumerate(digit_bits or [8] * N.TUNE_BUCKETS):
+            t.digit_bits[b] = int(d)
         self._check(self._lib.wahgpu_set_tuning(self._h, ctypes.byref(t)))
 
     @staticmethod
@@ -310,7 +313,8 @@ class Context:
         return {"bucket_log2": list(t.bucket_log2), "shape": list(t.shape),
                 "digit_bits": list(t.digit_bits),
                 "ms": [[round(x, 4) for x in row] for row in t.ms],
-                "shapes": [(int(a), int(b)) for a, b in zip(t.shape_threads, t.shape_items)]}
+                "shapes": [(int(a), int(b)) for a, b in zip(t.shape_threads, t.shape_items)],
+                "digit_ms": [[round(x, 4) for x in row] for row in t.digit_ms]}
 
     def graph_stats(self):
         cap, rep = ctypes.c_uint64(0), ctypes.c_uint64(0)

@@ -81,7 +81,8 @@ class Tuning(ctypes.Structure):
                 ("digit_bits", ctypes.c_uint32 * TUNE_BUCKETS),
                 ("ms", (ctypes.c_double * TUNE_SHAPES) * TUNE_BUCKETS),
                 ("shape_threads", ctypes.c_uint32 * TUNE_SHAPES),
-                ("shape_items", ctypes.c_uint32 * TUNE_SHAPES)]
+                ("shape_items", ctypes.c_uint32 * TUNE_SHAPES),
+                ("digit_ms", (ctypes.c_double * 2) * TUNE_BUCKETS)]
 
 
 # name -> (restype, argtypes); every function the header declares.

@@ -101,11 +101,26 @@ constexpr int kScanTile = 256 * kScanItems;
 constexpr uint32_t kHistMaxBlocks = 2048;
 // Tile counters: radix launches 0-3 of pair-mode columns, then their tail;
 // radix launches 0-3 of run-mode columns, then their tail.
-constexpr int kCounters = 11;
+constexpr int kCounters = 14;
 constexpr int kCounterTail = 4;
 constexpr int kCounterRun = 5;
 constexpr int kCounterRunTail = 9;
 constexpr int kCounterCut = 10;
+constexpr int kCounterWide = 11;  // wide-digit launches 0-2
+
+// Wide digits.  A pair-mode column whose keys take fewer passes of up to 11
+// bits than of 8 (9-11, 17-22 or 25-32 significant bits, trivial 8-bit
+// digits skipped) sorts with 2048-bin digits when its length bucket allows
+// it: a second histogram over its values (the first one only gives 8-bit
+// digits) and up to three passes with 2^bits bins each, the key's
+// significant bits split evenly between them.
+constexpr int kWideBits = 11;
+constexpr uint32_t kWideBins = 1u << kWideBits;
+constexpr int kWidePasses = 3;
+constexpr int kWideThreads = 512;
+constexpr int kWideItems = 16;
+constexpr uint32_t kWideTile = kWideThreads * kWideItems;
+constexpr uint64_t kWideMinValues = 1u << 14;  // shorter columns keep 8-bit digits
 
 // Which buffer a pass reads / writes.
 constexpr uint32_t kBufValues = 0;  // the input column, rids implicit (rid_base + i)
@@ -132,18 +147,23 @@ struct Column {
   uint32_t tail_tile0;
   uint32_t hist_chunk0;
   uint32_t index;  // slot of the column's counts in the caller's array
-  uint32_t pad[3];
+  uint32_t wide_slot;   // 1 + index of its wide-digit state; 0: 8-bit digits only
+  uint32_t wide_tile0;  // its first wide tile (prefix over the preceding columns)
+  uint32_t pad;
 };
 static_assert(sizeof(Column) == 48, "descriptor layout");
 
 struct Dims {
   uint32_t ncols, sort_tiles, tail_tiles, hist_chunks;
+  uint32_t wide_tiles;  // kWideTile tiles of the columns that may take wide digits
+  uint32_t pad;
 };
 // Device descriptor: dims, padding, then ncols Columns.
 struct DescHead {
   Dims dims;
-  uint32_t pad[4];
+  uint32_t pad[2];
 };
+static_assert(sizeof(DescHead) == 32, "descriptor layout");
 __host__ __device__ inline const Column* desc_cols(const DescHead* d) {
   return reinterpret_cast<const Column*>(d + 1);
 }
@@ -157,7 +177,8 @@ struct Header {
   uint32_t fault;                         // nonzero after a failed wait / bad index
   uint32_t max_active[2];                 // most radix passes of a column, per mode
   uint32_t mode_cols[2];                  // nonempty columns per mode
-  uint32_t pad[59];
+  uint32_t max_wide;                      // most wide-digit passes of a column
+  uint32_t pad[58];
 };
 
 // Per-column state.  hist and maxval are zeroed by k_begin; the rest is
@@ -169,6 +190,9 @@ struct Header {
 // (runs, low cardinality in order) shrink to a fraction of n items.
 constexpr uint32_t kModePair = 0;
 constexpr uint32_t kModeRun = 1;
+// counts.mode of a pair-mode column sorted with wide digits (ColState.mode
+// stays kModePair)
+constexpr uint32_t kModeWideReport = 2;
 
 struct ColState {
   uint32_t hist[4][256];          // histogram of each 8-bit digit
@@ -180,10 +204,20 @@ struct ColState {
   uint32_t mode;                  // kModePair / kModeRun
   uint32_t items;                 // items sorted: n, or the runs in run mode
   uint32_t narrow;                // 1: keys < 2^16 move as 16-bit words between passes
-  uint32_t pad[54];
+  uint32_t wide;                  // 1: sorts with wide digits (num_active of them)
+  uint32_t wide_slot;             // its WideState
+  uint32_t wbits[kWidePasses];    // wide digit widths
+  uint32_t wshift[kWidePasses];   // and shifts
+  uint32_t pad[46];
   uint32_t digit_offset[4][256];  // exclusive prefix of each item digit histogram
 };
 constexpr uint32_t kColZeroWords = 8 * 256 + 1;
+
+// Per column that may take wide digits: the wide digit histograms, turned
+// into exclusive prefixes (global digit offsets) by k_wplan.
+struct WideState {
+  uint32_t hist[kWidePasses][kWideBins];
+};
 
 struct BuildArgs {
   const uint32_t* values;            // concatenated input
@@ -198,6 +232,9 @@ struct BuildArgs {
   uint32_t* tile_entries;            // tail_tiles_cap words: entries (k) per tail tile
   uint32_t* tile_runs;               // sort_tiles_cap words: runs per radix tile, then
                                      // their exclusive prefix within the column
+  WideState* wide;                   // wide_slots_cap entries
+  uint32_t* wide_status;             // 2 arrays of wide_status_stride words
+  uint64_t wide_status_stride;
   uint32_t *out_keys, *out_offsets, *out_lengths, *out_words;
   uint32_t *sorted_keys, *sorted_rids;  // stage API sort output (one column)
   wahgpu_counts_dev* counts;         // per column
@@ -206,6 +243,7 @@ struct BuildArgs {
   // Upper bounds the grids are sized for (a captured graph replays any batch
   // within them; the kernels read the actual sizes from the descriptor).
   uint32_t cols_cap, sort_tiles_cap, tail_tiles_cap, hist_chunks_cap;
+  uint32_t wide_slots_cap, wide_tiles_cap;  // 0: no column may take wide digits
   uint64_t n_cap;
   // Every column sorts (value, rid) pairs (the stage API needs them).
   bool force_pair;

@@ -269,10 +269,12 @@ __global__ __launch_bounds__(kBlock) void k_plan(const DescHead* __restrict__ de
                                                  ColState* __restrict__ cs, Header* __restrict__ hdr,
                                                  uint32_t* __restrict__ tile_runs,
                                                  wahgpu_counts_dev* __restrict__ counts,
-                                                 int force_pair, uint32_t sort_tile) {
+                                                 int force_pair, uint32_t sort_tile,
+                                                 WideState* __restrict__ wide) {
   __shared__ uint64_t s_scan[kWaves];
   __shared__ int s_trivial[4];
   __shared__ uint32_t s_reps;
+  __shared__ uint32_t s_wide;  // 1 + wide slot of a column that takes wide digits
   const int t = threadIdx.x;
   const uint32_t ncols = desc->dims.ncols;
   for (uint32_t c = blockIdx.x; c < ncols; c += gridDim.x) {
@@ -350,6 +352,26 @@ __global__ __launch_bounds__(kBlock) void k_plan(const DescHead* __restrict__ de
       for (int p = 0; p < 4; ++p)
         if (n > 0 && !s_trivial[p]) col->shift[a++] = 8u * p;
       for (uint32_t q = a; q < 4; ++q) col->shift[q] = 0;
+      // Wide digits when they take fewer passes: the key's significant bits
+      // split as evenly as possible into passes of at most kWideBits.
+      const Column cw = desc_cols(desc)[c];
+      const uint32_t bits = col->maxval ? 32u - (uint32_t)__clz(col->maxval) : 0u;
+      const uint32_t aw = (bits + kWideBits - 1) / kWideBits;
+      const bool wide = cw.wide_slot != 0u && !run_mode && !force_pair && n >= kWideMinValues &&
+                        aw < a;
+      col->wide = wide ? 1u : 0u;
+      col->wide_slot = wide ? cw.wide_slot - 1u : 0u;
+      s_wide = wide ? cw.wide_slot : 0u;
+      if (wide) {
+        uint32_t at = 0;
+        for (uint32_t i = 0; i < (uint32_t)kWidePasses; ++i) {
+          const uint32_t b = i < aw ? bits / aw + (i < bits % aw ? 1u : 0u) : 0u;
+          col->wbits[i] = b;
+          col->wshift[i] = at;
+          at += b;
+        }
+        a = aw;
+      }
       col->num_active = a;
       col->mode = run_mode ? kModeRun : kModePair;
       // keys below 2^16 travel as 16-bit words from the first pass's output
@@ -359,12 +381,18 @@ __global__ __launch_bounds__(kBlock) void k_plan(const DescHead* __restrict__ de
       // pair mode: values, then A, B, A, B; run mode: the cut writes B
       if (run_mode) col->final_buf = (a % 2) ? kBufA : kBufB;
       else col->final_buf = (a == 0) ? kBufValues : ((a - 1) % 2 == 0 ? kBufA : kBufB);
-      if (a) atomicMax(&hdr->max_active[run_mode ? 1 : 0], a);
+      if (a) atomicMax(wide ? &hdr->max_wide : &hdr->max_active[run_mode ? 1 : 0], a);
       if (n) atomicAdd(&hdr->mode_cols[run_mode ? 1 : 0], 1u);
       wahgpu_counts_dev z;
       z.k = 0; z.length = 0; z.keycnt = 0;
-      z.bound = col->maxval; z.passes = a; z.fault = 0; z.mode = run_mode ? kModeRun : kModePair;
+      z.bound = col->maxval; z.passes = a; z.fault = 0;
+      z.mode = run_mode ? kModeRun : wide ? kModeWideReport : kModePair;
       counts[desc_cols(desc)[c].index] = z;
+    }
+    __syncthreads();
+    if (s_wide) {  // the wide histograms start at zero (k_whist adds to them)
+      uint32_t* hw = wide[s_wide - 1u].hist[0];
+      for (uint32_t i = t; i < (uint32_t)kWidePasses * kWideBins; i += kBlock) hw[i] = 0;
     }
     __syncthreads();
   }
@@ -645,7 +673,8 @@ __global__ __launch_bounds__(THREADS, sort_waves_per_eu(ITEMS)) void k_onesweep(
     // elements: rows, or the run items of a run-mode column
     const uint64_t n = RUN ? (uint64_t)col->items : cd.n;
     const uint64_t n_out = n;
-    if (col->mode != (RUN ? kModeRun : kModePair) || col->num_active <= a || base >= n) {
+    if (col->mode != (RUN ? kModeRun : kModePair) || col->wide || col->num_active <= a ||
+        base >= n) {
       // other mode, already sorted, or past the run items; a block draws its
       // tiles in increasing order, so past the last column's items it is done
       if (RUN && c + 1 == dims.ncols && col->mode == kModeRun && base >= n) break;
@@ -865,6 +894,347 @@ __global__ __launch_bounds__(THREADS, sort_waves_per_eu(ITEMS)) void k_onesweep(
     } else {
       dispatch(std::false_type{});
     }
+    __syncthreads();
+  }
+}
+
+#ifndef WAH_WIDE_LOOKAHEAD  // predecessors per look-back round trip of the wide passes
+#define WAH_WIDE_LOOKAHEAD 8
+#endif
+
+// ---------------------------------------------------------------------------
+// Wide digits (pair-mode columns that k_plan marked wide).
+//
+// k_whist: histograms of the column's wide digits over its values (the
+// 8-bit ones of k_histogram do not give them), in LDS per block, added to
+// the column's WideState; and zero look-back state for the first wide pass.
+// Blocks take contiguous ranges of the 4096-value histogram chunks, as in
+// k_histogram, and skip the chunks of other columns.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(kBlock) void k_whist(const uint32_t* __restrict__ values,
+                                                  const DescHead* __restrict__ desc,
+                                                  const ColState* __restrict__ cs,
+                                                  WideState* __restrict__ ws,
+                                                  uint32_t* __restrict__ wstatus0) {
+  __shared__ uint32_t h[kWidePasses * kWideBins];
+  const int t = threadIdx.x;
+  const Dims dims = desc->dims;
+  const Column* cols = desc_cols(desc);
+  for (uint32_t tile = blockIdx.x; tile < dims.wide_tiles; tile += gridDim.x) {
+    const uint32_t c = find_segment(&cols[0].wide_tile0, sizeof(Column) / 4, dims.ncols, tile);
+    if (!cs[c].wide) continue;  // block-uniform
+    uint32_t* st = wstatus0 + (uint64_t)tile * kWideBins;
+    for (uint32_t i = t; i < kWideBins; i += kBlock) st[i] = 0;
+  }
+  for (uint32_t i = t; i < (uint32_t)kWidePasses * kWideBins; i += kBlock) h[i] = 0;
+  __syncthreads();
+  const uint32_t per = (dims.hist_chunks + gridDim.x - 1) / gridDim.x;
+  const uint32_t c0 = blockIdx.x * per;
+  const uint32_t c1 = c0 + per < dims.hist_chunks ? c0 + per : dims.hist_chunks;
+  uint32_t col = c0 < c1 ? find_segment(&cols[0].hist_chunk0, sizeof(Column) / 4, dims.ncols, c0)
+                         : 0u;
+  bool counted = false;  // h holds counts of column `col`
+  auto flush = [&]() {
+    __syncthreads();
+    uint32_t* dst = ws[cs[col].wide_slot].hist[0];
+    for (uint32_t i = t; i < (uint32_t)kWidePasses * kWideBins; i += kBlock) {
+      if (h[i]) atomicAdd(&dst[i], h[i]);
+      h[i] = 0;
+    }
+    __syncthreads();
+    counted = false;
+  };
+  for (uint32_t c = c0; c < c1;) {
+    const uint32_t col_end = col + 1 < dims.ncols ? cols[col + 1].hist_chunk0 : dims.hist_chunks;
+    if (c >= col_end) {  // next column (block-uniform)
+      if (counted) flush();
+      col = find_segment(&cols[0].hist_chunk0, sizeof(Column) / 4, dims.ncols, c);
+      continue;
+    }
+    const ColState* st = &cs[col];
+    if (!st->wide) {  // skip the rest of this column's chunks
+      c = col_end;
+      continue;
+    }
+    counted = true;
+    const Column cd = cols[col];
+    const uint64_t lo = (uint64_t)(c - cd.hist_chunk0) * kHistChunk;
+    const uint32_t cnt = cd.n - lo < kHistChunk ? (uint32_t)(cd.n - lo) : kHistChunk;
+    const uint32_t* src = values + cd.off + lo;
+    const uint32_t np = st->num_active;
+    const uint32_t sh0 = st->wshift[0], sh1 = st->wshift[1], sh2 = st->wshift[2];
+    const uint32_t m0 = (1u << st->wbits[0]) - 1u, m1 = (1u << st->wbits[1]) - 1u,
+                   m2 = (1u << st->wbits[2]) - 1u;
+    auto add = [&](uint32_t v) {
+      atomicAdd(&h[(v >> sh0) & m0], 1u);
+      if (np > 1) atomicAdd(&h[kWideBins + ((v >> sh1) & m1)], 1u);
+      if (np > 2) atomicAdd(&h[2 * kWideBins + ((v >> sh2) & m2)], 1u);
+    };
+    if (cnt == kHistChunk && ((uintptr_t)src % 16) == 0) {
+      // four 16-byte loads per thread, issued before any is counted
+      const uint4* v4 = reinterpret_cast<const uint4*>(src);
+      uint4 q[4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) q[u] = v4[u * kBlock + t];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        add(q[u].x);
+        add(q[u].y);
+        add(q[u].z);
+        add(q[u].w);
+      }
+    } else {
+      for (uint32_t i = t; i < cnt; i += kBlock) add(src[i]);
+    }
+    ++c;
+  }
+  if (counted) flush();
+}
+
+// k_wplan: each wide column's digit histograms -> exclusive prefixes (the
+// first row of each digit's keys in the column), in place.
+__global__ __launch_bounds__(kBlock) void k_wplan(const DescHead* __restrict__ desc,
+                                                  const ColState* __restrict__ cs,
+                                                  WideState* __restrict__ ws) {
+  __shared__ uint64_t s_scan[kWaves];
+  constexpr uint32_t kPer = kWideBins / kBlock;
+  const int t = threadIdx.x;
+  const uint32_t ncols = desc->dims.ncols;
+  for (uint32_t c = blockIdx.x; c < ncols; c += gridDim.x) {
+    const ColState* col = &cs[c];
+    if (!col->wide) continue;  // block-uniform
+    for (uint32_t p = 0; p < col->num_active; ++p) {
+      uint32_t* hp = ws[col->wide_slot].hist[p] + (uint32_t)t * kPer;
+      uint32_t x[kPer];
+      uint64_t sum = 0;
+#pragma unroll
+      for (uint32_t k = 0; k < kPer; ++k) {
+        x[k] = hp[k];
+        sum += x[k];
+      }
+      uint64_t total;
+      uint64_t ex = block_exclusive<OpAdd>(sum, s_scan, &total);
+#pragma unroll
+      for (uint32_t k = 0; k < kPer; ++k) {
+        hp[k] = (uint32_t)ex;
+        ex += x[k];
+      }
+    }
+  }
+}
+
+// k_wsweep: launch `a` sorts the a-th wide digit of every wide column, one
+// tile of kWideTile pairs at a time, as k_onesweep does with 8-bit digits:
+// rank in LDS, publish the tile's digit counts, look back for the counts of
+// the tiles before it, scatter through LDS.  The 2048 digits take two 16-bit
+// counters per LDS word (a wave ranks at most 64 x kWideItems keys and a
+// tile holds 8192), and thread t owns digits 4t .. 4t + 3.
+template <int RANK>
+__global__ __launch_bounds__(kWideThreads, sort_waves_per_eu(kWideItems)) void k_wsweep(
+    uint32_t a, const uint32_t* __restrict__ values, uint32_t* keysA, uint32_t* ridsA,
+    uint32_t* keysB, uint32_t* ridsB, const DescHead* __restrict__ desc,
+    const ColState* __restrict__ cs, const WideState* __restrict__ ws, Header* __restrict__ hdr,
+    uint32_t* status_base, uint64_t status_stride, uint32_t lanes) {
+  constexpr int THREADS = kWideThreads, ITEMS = kWideItems, TILE = (int)kWideTile;
+  constexpr int WAVES = THREADS / kWave;
+  constexpr int DPT = (int)kWideBins / THREADS;  // digits per thread
+  static_assert(DPT == 4, "two counter words per thread and wave");
+  static_assert(TILE <= 65535, "16-bit counters and slots");
+  __shared__ uint32_t s_stage[TILE];
+  __shared__ uint32_t s_cnt[WAVES][kWideBins / 2];
+  __shared__ uint32_t s_glob[kWideBins];
+  __shared__ uint32_t s_wsum[WAVES];
+  __shared__ uint32_t s_tilen;
+  __shared__ uint32_t s_tile;
+
+  const bool implicit = a == 0;
+  const uint32_t* src_keys = implicit ? values : ((a % 2) ? keysA : keysB);
+  const uint32_t* src_rids = (a % 2) ? ridsA : ridsB;
+  uint32_t* dst_keys = (a % 2) ? keysB : keysA;
+  uint32_t* dst_rids = (a % 2) ? ridsB : ridsA;
+  uint32_t* status = status_base + (uint64_t)(a % 2) * status_stride;
+  uint32_t* status_next = status_base + (uint64_t)(1u - a % 2) * status_stride;
+  uint32_t* fault = &hdr->fault;
+  const Dims dims = desc->dims;
+  const Column* cols = desc_cols(desc);
+  const int t = threadIdx.x, l = t % kWave;
+  const int w = __builtin_amdgcn_readfirstlane(t / kWave);
+  if (hdr->max_wide <= a) return;  // no column has an a-th wide pass
+  uint32_t* counter = hdr->tiles[kCounterWide + a];
+
+  uint32_t iter = 0;
+  for (;;) {
+    for (int i = t; i < WAVES * (int)kWideBins / 2; i += THREADS) (&s_cnt[0][0])[i] = 0;
+    const uint32_t tile = __builtin_amdgcn_readfirstlane(
+        next_tile(counter, lanes, &s_tile, iter, WAH_SORT_TILE_GROUP));
+    if (tile >= dims.wide_tiles) break;
+    const uint32_t c = find_segment(&cols[0].wide_tile0, sizeof(Column) / 4, dims.ncols, tile);
+    const ColState* col = &cs[c];
+    const Column cd = cols[c];
+    const uint32_t first = cd.wide_tile0;
+    const uint64_t base = (uint64_t)(tile - first) * TILE;  // column-local
+    const uint64_t n = cd.n;
+    if (!col->wide || col->num_active <= a || base >= n) {
+      __syncthreads();  // s_tile is read by every wave before it is rewritten
+      continue;
+    }
+    const uint32_t shift = col->wshift[a];
+    const uint32_t dmask = (1u << col->wbits[a]) - 1u;
+    const uint32_t* goff = ws[col->wide_slot].hist[a];
+    const bool narrow = col->narrow != 0u, src_narrow = narrow && !implicit;
+#pragma unroll
+    for (int k = 0; k < DPT; ++k) status_next[(uint64_t)tile * kWideBins + DPT * t + k] = 0;
+    const uint64_t wbase = base + (uint64_t)w * kWave * ITEMS + l;
+    auto sweep = [&](auto full_tag) {
+      constexpr bool FULL = decltype(full_tag)::value;
+      uint32_t key[ITEMS], rank[ITEMS];
+      const uint32_t nvalid =
+          FULL ? (uint32_t)ITEMS * kWave : (uint32_t)(n > wbase ? n - wbase : 0);
+      if (src_narrow) {
+        const uint16_t* kp = reinterpret_cast<const uint16_t*>(src_keys + cd.off) + wbase;
+#pragma unroll
+        for (int i = 0; i < ITEMS; ++i)
+          key[i] = FULL || (uint32_t)(i * kWave) < nvalid ? (uint32_t)kp[i * kWave] : 0u;
+      } else {
+        const uint32_t* kp = src_keys + cd.off + wbase;
+#pragma unroll
+        for (int i = 0; i < ITEMS; ++i)
+          key[i] = FULL || (uint32_t)(i * kWave) < nvalid ? kp[i * kWave] : 0u;
+      }
+      uint32_t part = 0;
+#pragma unroll
+      for (int i = 0; i < ITEMS; ++i)
+        part |= (FULL || (uint32_t)(i * kWave) < nvalid ? 1u : 0u) << i;
+#pragma unroll
+      for (int i = 0; i < ITEMS; ++i) {
+        if (FULL || ((part >> i) & 1u)) {
+          const uint32_t d = (key[i] >> shift) & dmask;
+          const uint32_t hs = 16u * (d & 1u);
+          if (RANK == kRankAtomic) {
+            rank[i] = (atomicAdd(&s_cnt[w][d >> 1], 1u << hs) >> hs) & 0xFFFFu;
+          } else {
+            uint64_t peers = ballot(true);
+#pragma unroll
+            for (int b = 0; b < kWideBits; ++b) {
+              const bool bit = (d >> b) & 1u;
+              const uint64_t bb = ballot(bit);
+              peers &= bit ? bb : ~bb;
+            }
+            const uint32_t cnt = (uint32_t)__popcll(peers);
+            const uint32_t below = mbcnt(peers);
+            const uint32_t prior = (s_cnt[w][d >> 1] >> hs) & 0xFFFFu;
+            rank[i] = prior + below;
+            // two digits share a word: add, do not store
+            if (below == cnt - 1) atomicAdd(&s_cnt[w][d >> 1], cnt << hs);
+          }
+        }
+      }
+      __syncthreads();
+      // digits 4t .. 4t + 3: wave prefixes and tile totals
+      uint32_t tot[DPT] = {0u, 0u, 0u, 0u};
+#pragma unroll
+      for (int i = 0; i < WAVES; ++i) {
+        const uint32_t w0 = s_cnt[i][2 * t], w1 = s_cnt[i][2 * t + 1];
+        s_cnt[i][2 * t] = tot[0] | (tot[1] << 16);
+        s_cnt[i][2 * t + 1] = tot[2] | (tot[3] << 16);
+        tot[0] += w0 & 0xFFFFu;
+        tot[1] += w0 >> 16;
+        tot[2] += w1 & 0xFFFFu;
+        tot[3] += w1 >> 16;
+      }
+      uint32_t* my = &status[(uint64_t)tile * kWideBins + DPT * t];
+#pragma unroll
+      for (int k = 0; k < DPT; ++k)
+        st_agent(&my[k], ((tile == first ? kFlagPrefix : kFlagAgg) << 30) | tot[k]);
+      const uint32_t sum4 = tot[0] + tot[1] + tot[2] + tot[3];
+      const uint32_t inc = wave_incl_add(sum4);
+      if (l == kWave - 1) s_wsum[w] = inc;
+      __syncthreads();
+      uint32_t ex = inc - sum4;  // tile-local first slot of digit 4t
+#pragma unroll
+      for (int i = 0; i < WAVES; ++i) ex += i < w ? s_wsum[i] : 0u;
+      uint32_t exk[DPT];
+#pragma unroll
+      for (int k = 0; k < DPT; ++k) {
+        exk[k] = ex;
+        ex += tot[k];
+      }
+      if (t == THREADS - 1) s_tilen = ex;
+#pragma unroll
+      for (int i = 0; i < WAVES; ++i) {
+        s_cnt[i][2 * t] += exk[0] | (exk[1] << 16);
+        s_cnt[i][2 * t + 1] += exk[2] | (exk[3] << 16);
+      }
+      __syncthreads();
+      // Stage the keys in digit order before the look-back, which then
+      // holds few registers (the tile-local slots do not depend on it).
+#pragma unroll
+      for (int i = 0; i < ITEMS; ++i) {
+        if (FULL || ((part >> i) & 1u)) {
+          const uint32_t d = (key[i] >> shift) & dmask;
+          const uint32_t pos = ((s_cnt[w][d >> 1] >> (16u * (d & 1u))) & 0xFFFFu) + rank[i];
+          s_stage[pos] = key[i];
+          rank[i] = pos;  // keep the LDS slot for the rids
+        }
+      }
+      uint32_t slot2[(ITEMS + 1) / 2];
+#pragma unroll
+      for (int i = 0; i < ITEMS; i += 2)
+        slot2[i / 2] = (rank[i] & 0xFFFFu) | (i + 1 < ITEMS ? rank[i + 1 < ITEMS ? i + 1 : i] << 16 : 0u);
+      uint32_t prev[DPT] = {0u, 0u, 0u, 0u};
+      if (tile > first) {
+        lookback_digits4<WAH_WIDE_LOOKAHEAD>(status + DPT * t, tile, first, kWideBins, fault, prev);
+#pragma unroll
+        for (int k = 0; k < DPT; ++k) st_agent(&my[k], (kFlagPrefix << 30) | (prev[k] + tot[k]));
+      }
+#pragma unroll
+      for (int k = 0; k < DPT; ++k)
+        s_glob[DPT * t + k] = goff[DPT * t + k] + prev[k] - exk[k];  // + slot = column index
+      uint32_t rid[ITEMS];
+      if (implicit) {
+#pragma unroll
+        for (int i = 0; i < ITEMS; ++i) rid[i] = cd.rid_base + (uint32_t)wbase + (uint32_t)(i * kWave);
+      } else {
+        const uint32_t* rp = src_rids + cd.off + wbase;
+#pragma unroll
+        for (int i = 0; i < ITEMS; ++i)
+          rid[i] = FULL || (uint32_t)(i * kWave) < nvalid ? rp[i * kWave] : 0u;
+      }
+      __syncthreads();
+      const uint32_t tile_n = FULL ? (uint32_t)TILE : s_tilen;
+      uint32_t* dk = dst_keys + cd.off;
+      uint32_t* dr = dst_rids + cd.off;
+      uint16_t* dk16 = reinterpret_cast<uint16_t*>(dk);
+      const uint32_t last = (uint32_t)(n - 1);  // stores stay in the column
+      bool bad = false;
+      uint32_t gq[ITEMS];
+#pragma unroll
+      for (int j = 0; j < ITEMS; ++j) {
+        const uint32_t q = (uint32_t)j * THREADS + t;
+        gq[j] = last;
+        if (FULL || q < tile_n) {
+          const uint32_t k = s_stage[q];
+          const uint32_t g = s_glob[(k >> shift) & dmask] + q;
+          bad |= g > last;
+          gq[j] = g < last ? g : last;
+          if (narrow) dk16[gq[j]] = (uint16_t)k;
+          else dk[gq[j]] = k;
+        }
+      }
+      __syncthreads();
+#pragma unroll
+      for (int i = 0; i < ITEMS; ++i)
+        if (FULL || ((part >> i) & 1u)) s_stage[(slot2[i / 2] >> (16 * (i % 2))) & 0xFFFFu] = rid[i];
+      __syncthreads();
+#pragma unroll
+      for (int j = 0; j < ITEMS; ++j) {
+        const uint32_t q = (uint32_t)j * THREADS + t;
+        if (FULL || q < tile_n) dr[gq[j]] = s_stage[q];
+      }
+      if (ballot(bad) && l == 0) atomicOr(fault, 2u);  // a scatter index past the column
+    };
+    if (base + TILE <= n) sweep(std::true_type{});
+    else sweep(std::false_type{});
     __syncthreads();
   }
 }
@@ -1630,7 +2000,12 @@ constexpr uint32_t sort_blocks_per_cu() {
   return b ? b : 1u;
 }
 
-// The run cut and the four radix launches for one tile shape.
+// Wide-digit blocks per CU: two 512-thread blocks (4 waves per SIMD) with
+// ~72 KiB of LDS each (staged keys, 16-bit counters, digit offsets).
+constexpr uint32_t kWideBlocksPerCU = 2;
+
+// The run cut and the four radix launches for one tile shape (with the
+// wide-digit launches when a column may take them).
 template <int T, int I>
 static void launch_radix(const BuildArgs& a, hipStream_t s, LaunchHook* hook) {
   const uint32_t grid = cap_grid(a.sort_tiles_cap, WAH_STATIC_TILES
@@ -1639,6 +2014,17 @@ static void launch_radix(const BuildArgs& a, hipStream_t s, LaunchHook* hook) {
   hipLaunchKernelGGL(k_runcut, dim3(cap_grid(a.sort_tiles_cap, 2 * a.max_resident)), dim3(kBlock),
                      0, s, a.values, a.keysB, a.ridsB, a.litsB, a.desc, a.cs, a.hdr, a.tile_runs,
                      (uint32_t)(T * I));
+  // wide digits: the histograms of the wide columns' digits and their offsets
+  const uint32_t wgrid = cap_grid(a.wide_tiles_cap, WAH_STATIC_TILES ? a.wide_tiles_cap
+                                                                    : a.max_resident / 4 * kWideBlocksPerCU);
+  if (a.wide_tiles_cap) {
+    uint64_t hb = (a.n_cap + 4 * kHistChunk - 1) / (4 * kHistChunk);
+    if (hb > a.hist_chunks_cap) hb = a.hist_chunks_cap;
+    hipLaunchKernelGGL(k_whist, dim3(cap_grid((uint32_t)hb, kHistMaxBlocks)), dim3(kBlock), 0, s,
+                       a.values, a.desc, a.cs, a.wide, a.wide_status);
+    hipLaunchKernelGGL(k_wplan, dim3(cap_grid(a.cols_cap, 1024)), dim3(kBlock), 0, s, a.desc,
+                       a.cs, a.wide);
+  }
   if (hook) hook->mark(s);
   // each launch runs the a-th pass of pair-mode columns, then of run-mode
   // columns; a kernel with no such column returns at once
@@ -1658,6 +2044,16 @@ static void launch_radix(const BuildArgs& a, hipStream_t s, LaunchHook* hook) {
                          a.values, a.keysA, a.ridsA, a.keysB, a.ridsB, a.litsA, a.litsB, a.desc,
                          a.cs, a.hdr, a.sort_status, a.sort_status_stride, tile_lanes(grid));
     }
+    if (a.wide_tiles_cap && p < (uint32_t)kWidePasses) {
+      if (a.atomic_rank)
+        hipLaunchKernelGGL((k_wsweep<kRankAtomic>), dim3(wgrid), dim3(kWideThreads), 0, s, p,
+                           a.values, a.keysA, a.ridsA, a.keysB, a.ridsB, a.desc, a.cs, a.wide,
+                           a.hdr, a.wide_status, a.wide_status_stride, tile_lanes(wgrid));
+      else
+        hipLaunchKernelGGL((k_wsweep<kRankMatch>), dim3(wgrid), dim3(kWideThreads), 0, s, p,
+                           a.values, a.keysA, a.ridsA, a.keysB, a.ridsB, a.desc, a.cs, a.wide,
+                           a.hdr, a.wide_status, a.wide_status_stride, tile_lanes(wgrid));
+    }
     if (hook) hook->mark(s);
   }
 }
@@ -1672,7 +2068,7 @@ void launch_sort(const BuildArgs& a, hipStream_t s, LaunchHook* hook) {
                      a.tile_runs,
                      radix_tile(a.shape));
   hipLaunchKernelGGL(k_plan, dim3(cap_grid(a.cols_cap, 1024)), dim3(kBlock), 0, s, a.desc, a.cs,
-                     a.hdr, a.tile_runs, a.counts, (int)a.force_pair, radix_tile(a.shape));
+                     a.hdr, a.tile_runs, a.counts, (int)a.force_pair, radix_tile(a.shape), a.wide);
   switch (a.shape) {
 #define WAH_SHAPE_CASE(sh) \
   case sh: launch_radix<radix_threads(sh), radix_items(sh)>(a, s, hook); break;

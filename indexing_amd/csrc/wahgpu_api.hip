@@ -52,6 +52,11 @@ struct wahgpu_ctx {
   uint32_t tune_shape[WAHGPU_TUNE_BUCKETS] = {0, 0, 0, 0, 0};
   double tune_ms[WAHGPU_TUNE_BUCKETS][WAHGPU_TUNE_SHAPES] = {};
   int force_shape = -1;  // autotune runs: every column in this shape
+  // radix digit width per bucket: 8, or 11 = wide digits allowed (when they
+  // take fewer passes)
+  uint32_t tune_digits[WAHGPU_TUNE_BUCKETS] = {8, 8, 8, 8, 8};
+  double tune_digit_ms[WAHGPU_TUNE_BUCKETS][2] = {};
+  int force_digits = 0;  // autotune runs: 8 or 11 for every column
   char* noalloc = nullptr;  // this device's copy of the static scratch array
 };
 
@@ -170,13 +175,16 @@ T* take(wahgpu_ctx* c, uint64_t count) {
 struct Caps {
   uint64_t n;  // values of the concatenated input (its highest end)
   uint32_t cols, sort_tiles, tail_tiles, hist_chunks;
+  uint32_t wide_slots = 0, wide_tiles = 0;  // columns that may take wide digits, their tiles
   bool operator==(const Caps& o) const {
     return n == o.n && cols == o.cols && sort_tiles == o.sort_tiles &&
-           tail_tiles == o.tail_tiles && hist_chunks == o.hist_chunks;
+           tail_tiles == o.tail_tiles && hist_chunks == o.hist_chunks &&
+           wide_slots == o.wide_slots && wide_tiles == o.wide_tiles;
   }
   bool covers(const Caps& o) const {
     return n >= o.n && cols >= o.cols && sort_tiles >= o.sort_tiles &&
-           tail_tiles >= o.tail_tiles && hist_chunks >= o.hist_chunks;
+           tail_tiles >= o.tail_tiles && hist_chunks >= o.hist_chunks &&
+           wide_slots >= o.wide_slots && wide_tiles >= o.wide_tiles;
   }
 };
 // Column-length bucket of n: < 2^14, < 2^17, < 2^20, the rest.
@@ -190,9 +198,25 @@ int shape_for(const wahgpu_ctx* c, uint64_t n) {
   return c->force_shape >= 0 ? c->force_shape : (int)c->tune_shape[bucket_of(n)];
 }
 
-Caps caps_one(uint64_t n, int shape) {
-  return Caps{n, 1u, wah::sort_tiles_of(n, wah::radix_tile(shape)), wah::tail_tiles_of(n),
-              wah::hist_chunks_of(n)};
+// A column of n values may take wide digits (k_plan decides from its values).
+bool wide_ok(const wahgpu_ctx* c, uint64_t n) {
+  const uint32_t d = c->force_digits ? (uint32_t)c->force_digits : c->tune_digits[bucket_of(n)];
+  return d == 11u && n >= wah::kWideMinValues;
+}
+uint32_t wide_tiles_of(uint64_t n) { return (uint32_t)((n + wah::kWideTile - 1) / wah::kWideTile); }
+
+Caps caps_one(uint64_t n, int shape, bool wide = false) {
+  Caps k{n, 1u, wah::sort_tiles_of(n, wah::radix_tile(shape)), wah::tail_tiles_of(n),
+         wah::hist_chunks_of(n)};
+  if (wide) {
+    k.wide_slots = 1;
+    k.wide_tiles = wide_tiles_of(n);
+  }
+  return k;
+}
+// The fused build's caps for one column under the context's tuning.
+Caps caps_for(const wahgpu_ctx* c, uint64_t n) {
+  return caps_one(n, shape_for(c, n), wide_ok(c, n));
 }
 
 // Bytes of scratch a fused build needs (besides outputs), with `ndesc`
@@ -208,6 +232,8 @@ uint64_t build_scratch_bytes(const Caps& k, uint32_t ndesc = 1) {
   b += align_up((uint64_t)k.tail_tiles * 8 + 8);
   b += align_up((uint64_t)k.tail_tiles * 4 + 4);
   b += align_up((uint64_t)k.sort_tiles * 4 + 4);
+  b += align_up((uint64_t)k.wide_slots * sizeof(wah::WideState));
+  b += align_up(2ull * k.wide_tiles * wah::kWideBins * 4);
   return b;
 }
 
@@ -230,6 +256,11 @@ void carve_build(wahgpu_ctx* c, BuildArgs* a, const Caps& k, int shape) {
   a->tail_status = take<uint64_t>(c, (uint64_t)k.tail_tiles + 1);
   a->tile_entries = take<uint32_t>(c, (uint64_t)k.tail_tiles + 1);
   a->tile_runs = take<uint32_t>(c, (uint64_t)k.sort_tiles + 1);
+  a->wide = take<wah::WideState>(c, k.wide_slots);
+  a->wide_status_stride = (uint64_t)k.wide_tiles * wah::kWideBins;
+  a->wide_status = take<uint32_t>(c, 2 * a->wide_status_stride);
+  a->wide_slots_cap = k.wide_slots;
+  a->wide_tiles_cap = k.wide_tiles;
   a->max_resident = c->max_resident;
   a->atomic_rank = c->atomic_rank;
   a->shape = shape;
@@ -245,11 +276,13 @@ wah::DescHead* carve_desc(wahgpu_ctx* c, uint32_t cols) {
 }
 
 // A single column at offset 0: its descriptor travels in k_begin's arguments.
-void set_one(BuildArgs* a, uint64_t n, uint32_t rid_base) {
+// wide: it may take wide digits (slot 0, tiles from 0).
+void set_one(BuildArgs* a, uint64_t n, uint32_t rid_base, bool wide = false) {
   a->inline_desc = true;
   a->one_dims = wah::Dims{1u, wah::sort_tiles_of(n, wah::radix_tile(a->shape)),
-                          wah::tail_tiles_of(n), wah::hist_chunks_of(n)};
-  a->one = wah::Column{0, n, rid_base, 0u, 0u, 0u, 0u, {0u, 0u, 0u}};
+                          wah::tail_tiles_of(n), wah::hist_chunks_of(n),
+                          wide ? wide_tiles_of(n) : 0u, 0u};
+  a->one = wah::Column{0, n, rid_base, 0u, 0u, 0u, 0u, wide ? 1u : 0u, 0u, 0u};
 }
 
 // Scan scratch, zeroed before use: tile-counter lanes (kLaneWords words),
@@ -545,13 +578,13 @@ static int build_device_impl(wahgpu_ctx* c, const uint32_t* d_values, uint64_t n
     const uint64_t bump = c->bump;
     c->arena = scratch_base;
     c->bump = 0;
-    carve_build(c, &a, caps_one(n, shape_for(c, n)), shape_for(c, n));
+    carve_build(c, &a, caps_for(c, n), shape_for(c, n));
     c->arena = arena;
     c->bump = bump;
   } else {
-    carve_build(c, &a, caps_one(n, shape_for(c, n)), shape_for(c, n));
+    carve_build(c, &a, caps_for(c, n), shape_for(c, n));
   }
-  set_one(&a, n, rid_base);
+  set_one(&a, n, rid_base, wide_ok(c, n));
   a.values = d_values;
   a.out_keys = d_keys;
   a.out_offsets = d_offsets;
@@ -617,7 +650,7 @@ int wahgpu_build_device(wahgpu_ctx* c, const uint32_t* d_values, uint64_t n, uin
   int rc = check_n(c, n, rid_base);
   if (rc) return rc;
   HIPCHK(c, hipSetDevice(c->device));
-  rc = arena_reserve(c, build_scratch_bytes(caps_one(n, shape_for(c, n))));
+  rc = arena_reserve(c, build_scratch_bytes(caps_for(c, n)));
   if (rc) return rc;
   return build_device_impl(c, d_values, n, rid_base, d_keys, d_offsets, d_lengths, d_words,
                            d_counts, nullptr);
@@ -638,7 +671,7 @@ int wahgpu_build_device_noalloc(wahgpu_ctx* c, const uint32_t* d_values, uint64_
   if (rc) return rc;
   if (!c->noalloc) return fail(c, WAHGPU_EHIP, "no static scratch on this device");
   // every tile shape fits: 24 bytes per value plus per-tile state
-  if (build_scratch_bytes(caps_one(n, shape_for(c, n))) > kNoallocScratchBytes)
+  if (build_scratch_bytes(caps_for(c, n)) > kNoallocScratchBytes)
     return fail(c, WAHGPU_ERANGE, "wahgpu_build_device_noalloc: scratch of " +
                                       std::to_string(kNoallocScratchBytes) + " bytes too small");
   HIPCHK(c, hipSetDevice(c->device));
@@ -658,7 +691,7 @@ int wahgpu_build(wahgpu_ctx* c, const uint32_t* values, uint64_t n, uint32_t rid
   if (rc) return rc;
   HIPCHK(c, hipSetDevice(c->device));
   uint64_t nn = n ? n : 1;
-  uint64_t bytes = build_scratch_bytes(caps_one(n, shape_for(c, n))) + align_up(nn * 4) * 4 + align_up(2 * nn * 4) +
+  uint64_t bytes = build_scratch_bytes(caps_for(c, n)) + align_up(nn * 4) * 4 + align_up(2 * nn * 4) +
                    align_up(sizeof(wahgpu_counts));
   rc = arena_reserve(c, bytes);
   if (rc) return rc;
@@ -752,24 +785,34 @@ int plan_batch(wahgpu_ctx* c, const wahgpu_column* cols, uint32_t ncols, uint64_
     sb.desc.assign(sizeof(wah::DescHead) + (uint64_t)m * sizeof(wah::Column), 0);
     auto* head = reinterpret_cast<wah::DescHead*>(sb.desc.data());
     wah::Column* dc = wah::desc_cols(head);
-    uint64_t st = 0, tt = 0, hc = 0;
+    uint64_t st = 0, tt = 0, hc = 0, wt = 0;
+    uint32_t ws = 0;  // columns that may take wide digits, each with a WideState
     const uint32_t tile = wah::radix_tile(sh);
     for (uint32_t j = 0; j < m; ++j) {
       const wahgpu_column& col = cols[members[sh][j]];
+      const bool wide = wide_ok(c, col.n);
       dc[j] = wah::Column{col.offset, col.n, col.rid_base, (uint32_t)st, (uint32_t)tt,
-                          (uint32_t)hc, members[sh][j], {0u, 0u, 0u}};
+                          (uint32_t)hc, members[sh][j], wide ? ws + 1u : 0u, (uint32_t)wt, 0u};
       st += wah::sort_tiles_of(col.n, tile);
       tt += wah::tail_tiles_of(col.n);
       hc += wah::hist_chunks_of(col.n);
+      if (wide) {
+        ++ws;
+        wt += wide_tiles_of(col.n);
+      }
     }
     if (st > 0xFFFFFFF0ull || hc > 0xFFFFFFF0ull)
       return fail(c, WAHGPU_ERANGE, "batch too large for 32-bit tile numbering");
-    head->dims = wah::Dims{m, (uint32_t)st, (uint32_t)tt, (uint32_t)hc};
+    head->dims = wah::Dims{m, (uint32_t)st, (uint32_t)tt, (uint32_t)hc, (uint32_t)wt, 0u};
     sb.caps = Caps{values_len, m, (uint32_t)st, (uint32_t)tt, (uint32_t)hc};
+    sb.caps.wide_slots = ws;
+    sb.caps.wide_tiles = (uint32_t)wt;
     shared->cols = std::max(shared->cols, m);
     shared->sort_tiles = std::max(shared->sort_tiles, (uint32_t)st);
     shared->tail_tiles = std::max(shared->tail_tiles, (uint32_t)tt);
     shared->hist_chunks = std::max(shared->hist_chunks, (uint32_t)hc);
+    shared->wide_slots = std::max(shared->wide_slots, ws);
+    shared->wide_tiles = std::max(shared->wide_tiles, (uint32_t)wt);
     subs->push_back(std::move(sb));
   }
   return WAHGPU_OK;
@@ -799,9 +842,16 @@ Caps reserved_caps(const wahgpu_ctx* c) {
   const uint64_t v = c->res_values;
   uint32_t min_tile = wah::radix_tile(0);
   for (int sh = 1; sh < wah::kRadixShapes; ++sh) min_tile = std::min(min_tile, wah::radix_tile(sh));
-  return Caps{v, c->res_cols, (uint32_t)(v / min_tile + c->res_cols),
-              (uint32_t)(v / wah::kTailTile + c->res_cols),
-              (uint32_t)(v / wah::kHistChunk + c->res_cols)};
+  Caps k{v, c->res_cols, (uint32_t)(v / min_tile + c->res_cols),
+         (uint32_t)(v / wah::kTailTile + c->res_cols),
+         (uint32_t)(v / wah::kHistChunk + c->res_cols)};
+  bool wide = false;  // some bucket allows wide digits
+  for (uint32_t b = 1; b < WAHGPU_TUNE_BUCKETS; ++b) wide |= c->tune_digits[b] == 11u;
+  if (wide) {
+    k.wide_slots = (uint32_t)std::min<uint64_t>(c->res_cols, v / wah::kWideMinValues);
+    k.wide_tiles = (uint32_t)(v / wah::kWideTile + k.wide_slots);
+  }
+  return k;
 }
 
 // Scratch bytes of a planned batch (shared scratch + one descriptor region
@@ -1039,8 +1089,10 @@ int wahgpu_get_tuning(const wahgpu_ctx* c, wahgpu_tuning* out) {
   for (uint32_t b = 0; b < WAHGPU_TUNE_BUCKETS; ++b) {
     out->bucket_log2[b] = kBucketLog2[b];
     out->shape[b] = c->tune_shape[b];
-    out->digit_bits[b] = 8;
+    out->digit_bits[b] = c->tune_digits[b];
     for (uint32_t s = 0; s < WAHGPU_TUNE_SHAPES; ++s) out->ms[b][s] = c->tune_ms[b][s];
+    out->digit_ms[b][0] = c->tune_digit_ms[b][0];
+    out->digit_ms[b][1] = c->tune_digit_ms[b][1];
   }
   for (int sh = 0; sh < WAHGPU_TUNE_SHAPES; ++sh) {
     out->shape_threads[sh] = (uint32_t)wah::radix_threads(sh);
@@ -1062,11 +1114,17 @@ static_assert(WAHGPU_TUNE_SHAPES == wah::kRadixShapes, "public shape count");
 
 int wahgpu_set_tuning(wahgpu_ctx* c, const wahgpu_tuning* in) {
   if (!c || !in) return WAHGPU_EINVAL;
-  for (uint32_t b = 0; b < WAHGPU_TUNE_BUCKETS; ++b)
+  for (uint32_t b = 0; b < WAHGPU_TUNE_BUCKETS; ++b) {
     if (in->shape[b] >= WAHGPU_TUNE_SHAPES) return fail(c, WAHGPU_EINVAL, "unknown tile shape");
+    if (in->digit_bits[b] != 0 && in->digit_bits[b] != 8 && in->digit_bits[b] != 11)
+      return fail(c, WAHGPU_EINVAL, "digit width must be 8 or 11");
+  }
   HIPCHK(c, hipSetDevice(c->device));
   drop_graph(c);  // the set of launch sets a batch runs may change
-  for (uint32_t b = 0; b < WAHGPU_TUNE_BUCKETS; ++b) c->tune_shape[b] = in->shape[b];
+  for (uint32_t b = 0; b < WAHGPU_TUNE_BUCKETS; ++b) {
+    c->tune_shape[b] = in->shape[b];
+    c->tune_digits[b] = in->digit_bits[b] == 11 ? 11u : 8u;
+  }
   return WAHGPU_OK;
 }
 
@@ -1096,6 +1154,7 @@ int wahgpu_autotune(wahgpu_ctx* c, wahgpu_tuning* out) {
     for (auto& e : ev)
       if (e) (void)hipEventDestroy(e);
     c->force_shape = -1;
+    c->force_digits = 0;
   };
   if (hipMalloc(&d_v, total * 4) != hipSuccess || hipMalloc(&d_k, total * 4) != hipSuccess ||
       hipMalloc(&d_o, total * 4) != hipSuccess || hipMalloc(&d_l, total * 4) != hipSuccess ||
@@ -1106,6 +1165,45 @@ int wahgpu_autotune(wahgpu_ctx* c, wahgpu_tuning* out) {
     cleanup();
     return fail(c, WAHGPU_ENOMEM, "autotune: device allocation failed");
   }
+  // best of three timed builds (after one warm-up) of the batch `cols`; a
+  // build that reported a fault is not a valid timing and fails the tune
+  std::string fault_msg;
+  auto time_batch = [&](const std::vector<wahgpu_column>& cols, uint64_t values,
+                        std::vector<wahgpu_counts>& hc, float* best_ms) {
+    const uint32_t ncols = (uint32_t)cols.size();
+    *best_ms = 1e30f;
+    for (int rep = 0; rep < 4 && rc == WAHGPU_OK; ++rep) {
+      (void)hipEventRecord(ev[0], c->stream);
+      rc = batch_device_impl(c, d_v, values, cols.data(), ncols, d_k, d_o, d_l, d_w, d_c, 0);
+      (void)hipEventRecord(ev[1], c->stream);
+      if (rc == WAHGPU_OK && hipEventSynchronize(ev[1]) != hipSuccess) rc = WAHGPU_EHIP;
+      float ms = 0.f;
+      if (rc == WAHGPU_OK) (void)hipEventElapsedTime(&ms, ev[0], ev[1]);
+      if (rc == WAHGPU_OK &&
+          hipMemcpy(hc.data(), d_c, ncols * sizeof(wahgpu_counts), hipMemcpyDeviceToHost) !=
+              hipSuccess)
+        rc = WAHGPU_EHIP;
+      for (uint32_t i = 0; rc == WAHGPU_OK && i < ncols; ++i)
+        if (hc[i].fault) {
+          fault_msg = "autotune: shape " + std::to_string(c->force_shape) + ", digits " +
+                      std::to_string(c->force_digits) + " reported fault " +
+                      std::to_string(hc[i].fault);
+          rc = WAHGPU_EHIP;
+        }
+      if (rc == WAHGPU_OK && rep > 0 && ms < *best_ms) *best_ms = ms;  // first run warms up
+    }
+  };
+  auto generate = [&](uint64_t values, uint32_t to, uint64_t seed) {
+    wahgpu_gen_params gp{};
+    gp.dist = WAHGPU_DIST_UNIFORM;
+    gp.from = 0;
+    gp.to = to;
+    gp.seed = seed;
+    if (wahgpu_generate(&gp, host.data(), values) != WAHGPU_OK) rc = WAHGPU_EINVAL;
+    if (rc == WAHGPU_OK &&
+        hipMemcpy(d_v, host.data(), values * 4, hipMemcpyHostToDevice) != hipSuccess)
+      rc = WAHGPU_EHIP;
+  };
   for (uint32_t b = 0; b < WAHGPU_TUNE_BUCKETS && rc == WAHGPU_OK; ++b) {
     const uint64_t len = (uint64_t)std::pow(2.0, len_log2[b]);
     const uint64_t step = (len + 63) / 64 * 64;
@@ -1114,47 +1212,44 @@ int wahgpu_autotune(wahgpu_ctx* c, wahgpu_tuning* out) {
     std::vector<wahgpu_column> cols(ncols);
     std::vector<wahgpu_counts> hc(ncols);
     for (uint32_t i = 0; i < ncols; ++i) cols[i] = wahgpu_column{i * step, len, 0u, 0u};
-    wahgpu_gen_params gp{};
-    gp.dist = WAHGPU_DIST_UNIFORM;
-    gp.from = 0;
-    gp.to = 4095;
-    gp.seed = 1000 + b;
-    if (wahgpu_generate(&gp, host.data(), values) != WAHGPU_OK) rc = WAHGPU_EINVAL;
-    if (rc == WAHGPU_OK &&
-        hipMemcpy(d_v, host.data(), values * 4, hipMemcpyHostToDevice) != hipSuccess)
-      rc = WAHGPU_EHIP;
+    // tile shape, with 8-bit digits, on keys of 12 bits
+    generate(values, 4095, 1000 + b);
     double best = 1e30;
+    c->force_digits = 8;
     for (int sh = 0; sh < WAHGPU_TUNE_SHAPES && rc == WAHGPU_OK; ++sh) {
       c->force_shape = sh;
-      float ms_best = 1e30f;
-      for (int rep = 0; rep < 4 && rc == WAHGPU_OK; ++rep) {
-        (void)hipEventRecord(ev[0], c->stream);
-        rc = batch_device_impl(c, d_v, values, cols.data(), ncols, d_k, d_o, d_l, d_w, d_c, 0);
-        (void)hipEventRecord(ev[1], c->stream);
-        if (rc == WAHGPU_OK && hipEventSynchronize(ev[1]) != hipSuccess) rc = WAHGPU_EHIP;
-        float ms = 0.f;
-        if (rc == WAHGPU_OK) (void)hipEventElapsedTime(&ms, ev[0], ev[1]);
-        // a build that reported a fault is not a valid timing: fail the tune
-        if (rc == WAHGPU_OK &&
-            hipMemcpy(hc.data(), d_c, ncols * sizeof(wahgpu_counts), hipMemcpyDeviceToHost) !=
-                hipSuccess)
-          rc = WAHGPU_EHIP;
-        for (uint32_t i = 0; rc == WAHGPU_OK && i < ncols; ++i)
-          if (hc[i].fault) {
-            cleanup();
-            return fail(c, WAHGPU_EHIP, "autotune: shape " + std::to_string(sh) +
-                                            " reported fault " + std::to_string(hc[i].fault));
-          }
-        if (rep > 0 && ms < ms_best) ms_best = ms;  // first run warms up
-      }
-      c->tune_ms[b][sh] = ms_best;
-      if (rc == WAHGPU_OK && ms_best < best) {
-        best = ms_best;
+      float ms = 0.f;
+      time_batch(cols, values, hc, &ms);
+      c->tune_ms[b][sh] = ms;
+      if (rc == WAHGPU_OK && ms < best) {
+        best = ms;
         c->tune_shape[b] = (uint32_t)sh;
       }
     }
+    // digit width in that shape, on keys of 10 bits (one wide pass against
+    // two 8-bit ones); columns too short for wide digits keep 8 bits
+    c->tune_digits[b] = 8;
+    c->tune_digit_ms[b][0] = c->tune_digit_ms[b][1] = 0.0;
+    if (rc == WAHGPU_OK && len >= wah::kWideMinValues) {
+      generate(values, 1023, 2000 + b);
+      c->force_shape = (int)c->tune_shape[b];
+      float ms8 = 0.f, ms11 = 0.f;
+      c->force_digits = 8;
+      time_batch(cols, values, hc, &ms8);
+      c->force_digits = 11;
+      if (rc == WAHGPU_OK) time_batch(cols, values, hc, &ms11);
+      for (uint32_t i = 0; rc == WAHGPU_OK && i < ncols; ++i)
+        if (hc[i].mode != wah::kModeWideReport) {
+          fault_msg = "autotune: a 10-bit column did not take wide digits";
+          rc = WAHGPU_EHIP;
+        }
+      c->tune_digit_ms[b][0] = ms8;
+      c->tune_digit_ms[b][1] = ms11;
+      if (rc == WAHGPU_OK && ms11 < ms8) c->tune_digits[b] = 11;
+    }
   }
   cleanup();
+  if (rc != WAHGPU_OK && !fault_msg.empty()) return fail(c, rc, fault_msg);
   if (rc != WAHGPU_OK) return rc == WAHGPU_EHIP ? fail(c, rc, "autotune: build failed") : rc;
   return out ? wahgpu_get_tuning(c, out) : WAHGPU_OK;
 }

@@ -167,3 +167,20 @@ def test_decoder_known_words():
     assert s == "101" + "0" * 28 + "0" * 62 + "0" * 30 + "1"
     assert decode_rows(words) == [0, 2, 31 * 3 + 30]
     assert oracle.decoded_bitmap(np.array([0x80000000], np.uint32)) == ""
+
+
+def test_max_size_closed_form_matches_oracle_on_small_cases():
+    """The closed form tests/test_gpu_maxsize.py checks 2^30-value builds
+    against, pinned here against the oracle on small columns (partial first
+    and last chunks, rid_base not a multiple of 31)."""
+    from tests.test_gpu_maxsize import expected_words
+    keys = [40, 10, 70, 20, 60, 30, 50]
+    for n, rb in ((10000, 0), (9999, 1000003), (217 * 3 + 5, 62), (31 * 7, 30)):
+        vals = np.array([keys[i % 7] for i in range(n)], np.uint32)
+        ref = oracle.build(vals, rid_base=rb)
+        pos = 0
+        for i, k in enumerate(sorted(keys)):
+            w = expected_words(n, rb, keys.index(k))
+            assert ref["keys"][i] == k and ref["offsets"][i] == pos and ref["lengths"][i] == w.size
+            assert np.array_equal(ref["words"][pos:pos + w.size], w), (n, rb, k)
+            pos += w.size

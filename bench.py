@@ -80,7 +80,7 @@ def byte_model(n, k, L, K, P, bound, mode):
     and writes 8n; the whole build is B_alg = 16nP + 8n + 4L + 12K (histogram
     4n, passes, the tail's 8n read, words 4L, keys, offsets and lengths 12K).
 
-    moved_pass / b_moved are the bytes this build moves: keys below 2^16
+    moved_pass / b_moved are the bytes the fused build moves: keys below 2^16
     travel between passes as 2-byte words (kb = 2, else 4); a run-mode column
     sorts one (key, rid, literal bits) item per run, at least k of them, cut
     from a second read of the values.
@@ -537,7 +537,7 @@ def golden_record(column, n):
 def kernel_table(group_ms, calls, bm, n, k, L, K, mode):
     """Per kernel group: mean ms, algorithmic bytes (the canonical byte model:
     histogram 4n, passes 12n / 16n, tail 8n + 4L + 8K, lengths 4K, which sum
-    to B_alg) and the bytes this build moves, each as GB/s."""
+    to B_alg) and the bytes the fused build moves, each as GB/s."""
     kb = bm["kb"]
     names = ["histogram", "pass0", "pass1", "pass2", "pass3", "tail", "lengths"]
     pass_ms = [group_ms[1 + p] / calls for p in range(4)]
@@ -587,7 +587,7 @@ def main():
         cpu_baseline_worker(*args.cpu_baseline_worker)
         return
     if args.gpus != 1:
-        print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
+        print(f"bench.py: runs on one GPU; --gpus {args.gpus} is not supported "
               "(the multi-GPU sharding is a design note only)", file=sys.stderr)
         sys.exit(2)
 

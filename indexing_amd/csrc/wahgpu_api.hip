@@ -91,7 +91,8 @@ int fail(wahgpu_ctx* c, int code, const std::string& msg) {
 // kernels never overlap.  Each launch sequence, from any context and on any
 // stream, waits for a per-device event that the previous sequence recorded
 // and records it after itself; the host lock keeps wait, launches and record
-// of one sequence together.  Copies and other kernels still overlap them.
+// of one sequence together (host waits come after end()).  Copies and other
+// kernels still overlap them.
 // Sequences enqueued while their stream is being captured into a graph are
 // not ordered: a caller replays such graphs on one stream at a time.
 constexpr int kOrderDevices = 64;
@@ -117,10 +118,15 @@ class DeviceOrder {
     if (st_ == hipSuccess) st_ = hipStreamWaitEvent(c->stream, slot_->last, 0);
     record_ = st_ == hipSuccess;
   }
-  ~DeviceOrder() {
+  ~DeviceOrder() { end(); }
+  // Records the event after the sequence and releases the lock; called
+  // before the caller synchronizes its stream, so no host wait ever holds
+  // the lock (a stream may wait for work that another thread enqueues).
+  void end() {
     if (!slot_) return;
     if (record_) (void)hipEventRecord(slot_->last, c_->stream);
     slot_->mu.unlock();
+    slot_ = nullptr;
   }
   DeviceOrder(const DeviceOrder&) = delete;
   DeviceOrder& operator=(const DeviceOrder&) = delete;
@@ -923,7 +929,6 @@ int batch_device_impl(wahgpu_ctx* c, const uint32_t* d_values, uint64_t values_l
     if (rc) return rc;
     args.push_back(a);
   }
-  ORDERED(c);
   if (graph) {
     std::vector<uintptr_t> key = {(uintptr_t)c->arena, (uintptr_t)c->stream,
                                   (uintptr_t)d_values, (uintptr_t)d_keys,
@@ -953,10 +958,12 @@ int batch_device_impl(wahgpu_ctx* c, const uint32_t* d_values, uint64_t values_l
       c->gkey = key;
       ++c->captures;
     }
+    ORDERED(c);  // after the capture (which enqueues nothing) and its synchronization
     HIPCHK(c, hipGraphLaunch(c->gexec, c->stream));
     ++c->replays;
     return WAHGPU_OK;
   }
+  ORDERED(c);
   for (const BuildArgs& a : args) {
     EventHook hook;
     EventHook* hp = nullptr;
@@ -1308,34 +1315,43 @@ int wahgpu_build_staged(wahgpu_ctx* c, const uint32_t* values, uint64_t n, uint3
   a.force_pair = true;  // the stages below take one (value, rid) pair per row
   // The passes ping-pong between (keysA, ridsA) and (keysB, ridsB); finalize
   // copies whichever holds the result into (s_keys, s_rids).
-  ORDERED(c);
-  wah::launch_sort(a, c->stream);
-  wah::launch_sort_finalize(a, c->stream);
-  LAUNCHCHK(c);
-  m.mark(c->stream);  // 2 sort
-  wah::launch_chunks(s_rids, n, chids, lits, c->stream);
-  LAUNCHCHK(c);
-  m.mark(c->stream);  // 3 chunks + literals
-  wah::launch_fill32(sm.base, sm.words, 0, c->stream);
-  wah::launch_merge(sm.args, s_keys, chids, lits, n, mv, mc, ml, sm.result, c->stream);
-  LAUNCHCHK(c);
+  // each segment of launches up to a host read is one ordered sequence
+  {
+    ORDERED(c);
+    wah::launch_sort(a, c->stream);
+    wah::launch_sort_finalize(a, c->stream);
+    LAUNCHCHK(c);
+    m.mark(c->stream);  // 2 sort
+    wah::launch_chunks(s_rids, n, chids, lits, c->stream);
+    LAUNCHCHK(c);
+    m.mark(c->stream);  // 3 chunks + literals
+    wah::launch_fill32(sm.base, sm.words, 0, c->stream);
+    wah::launch_merge(sm.args, s_keys, chids, lits, n, mv, mc, ml, sm.result, c->stream);
+    LAUNCHCHK(c);
+  }
   uint64_t k = 0;
   rc = read_scan_result(c, sm.header, &k);  // k sizes the next launches (src/phases.cpp:890-903)
   if (rc) return rc;
   m.mark(c->stream);  // 4 merge
-  wah::launch_fills(mv, mc, k, fills, c->stream);
-  LAUNCHCHK(c);
-  m.mark(c->stream);  // 5 fills
-  wah::launch_fill32(sf.base, sf.words, 0, c->stream);
-  wah::launch_compact(sf.args, fills, ml, k, 1, index, sf.result, c->stream);
-  LAUNCHCHK(c);
+  {
+    ORDERED(c);
+    wah::launch_fills(mv, mc, k, fills, c->stream);
+    LAUNCHCHK(c);
+    m.mark(c->stream);  // 5 fills
+    wah::launch_fill32(sf.base, sf.words, 0, c->stream);
+    wah::launch_compact(sf.args, fills, ml, k, 1, index, sf.result, c->stream);
+    LAUNCHCHK(c);
+  }
   uint64_t L = 0;
   rc = read_scan_result(c, sf.header, &L);
   if (rc) return rc;
   m.mark(c->stream);  // 6 fuse
-  wah::launch_fill32(sc.base, sc.words, 0, c->stream);
-  wah::launch_colen(sc.args, mv, fills, k, o_keys, o_offsets, o_lengths, d_counts, c->stream);
-  LAUNCHCHK(c);
+  {
+    ORDERED(c);
+    wah::launch_fill32(sc.base, sc.words, 0, c->stream);
+    wah::launch_colen(sc.args, mv, fills, k, o_keys, o_offsets, o_lengths, d_counts, c->stream);
+    LAUNCHCHK(c);
+  }
   wahgpu_counts cc;
   uint64_t unused;
   rc = read_scan_result(c, sc.header, &unused);  // fault word of the column kernel
@@ -1430,6 +1446,7 @@ int wahgpu_merge_lit_by_val_chids(wahgpu_ctx* c, const uint32_t* d_values,
   wah::launch_merge(s.args, d_values, d_chids, d_lits, n, d_values_out, d_chids_out, d_lits_out,
                     s.result, c->stream);
   LAUNCHCHK(c);
+  order_guard_.end();  // no host wait holds the device order
   return read_scan_result(c, s.header, k);
 }
 
@@ -1457,6 +1474,7 @@ int wahgpu_fuse_fill_literals(wahgpu_ctx* c, const uint32_t* d_fills, const uint
   wah::launch_fill32(s.base, s.words, 0, c->stream);
   wah::launch_compact(s.args, d_fills, d_lits, k, 1, d_index, s.result, c->stream);
   LAUNCHCHK(c);
+  order_guard_.end();  // no host wait holds the device order
   return read_scan_result(c, s.header, index_length);
 }
 
@@ -1482,6 +1500,7 @@ int wahgpu_compute_column_length(wahgpu_ctx* c, const uint32_t* d_values,
   LAUNCHCHK(c);
   wahgpu_counts cc;
   uint64_t unused;
+  order_guard_.end();  // no host wait holds the device order
   rc = read_scan_result(c, s.header, &unused);
   if (rc) return rc;
   rc = read_counts(c, d_counts, &cc);
@@ -1504,6 +1523,7 @@ int wahgpu_stream_compaction(wahgpu_ctx* c, const uint32_t* d_data, const uint32
   wah::launch_fill32(s.base, s.words, 0, c->stream);
   wah::launch_compact(s.args, d_data, d_valid, n, 0, d_out, s.result, c->stream);
   LAUNCHCHK(c);
+  order_guard_.end();  // no host wait holds the device order
   return read_scan_result(c, s.header, count);
 }
 
@@ -1596,6 +1616,7 @@ int wahgpu_decode_device(wahgpu_ctx* c, const uint32_t* d_words, uint64_t len, u
   wah::launch_decode(sc.args, d_words, nullptr, len, d_bitmap, nrows, d_last, c->stream);
   LAUNCHCHK(c);
   uint64_t unused;
+  order_guard_.end();  // no host wait holds the device order
   rc = read_scan_result(c, sc.header, &unused);  // fault word (synchronizes)
   if (rc) return rc;
   if (last_row) {
@@ -1643,6 +1664,7 @@ int wahgpu_query_range_device(wahgpu_ctx* c, const uint32_t* d_keys, const uint3
   wah::launch_decode(sc.args, d_words + w0, heads, len, d_bitmap, nrows, nullptr, c->stream);
   LAUNCHCHK(c);
   uint64_t unused;
+  order_guard_.end();  // no host wait holds the device order
   return read_scan_result(c, sc.header, &unused);
 }
 
@@ -1700,6 +1722,7 @@ int wahgpu_combine_device(wahgpu_ctx* c, int op, const uint32_t* d_a, uint64_t l
   wah::launch_encode(bm_a, bm_b, op, nrows, tile_sum, d_out, cap, d_len, c->stream);
   LAUNCHCHK(c);
   uint64_t unused;
+  order_guard_.end();  // no host wait holds the device order
   rc = read_scan_result(c, sa.header, &unused);
   if (rc) return rc;
   rc = read_scan_result(c, sb.header, &unused);

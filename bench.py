@@ -30,7 +30,9 @@ Beside it:
                 program) on the whole 2^28 column, run in a child process
                 pinned to one core while the GPU legs run, with per-stage
                 times, the CPU model and core count, and the literal bubble
-                sort checked against the stable sort on a 2^16 sample
+                sort checked against the stable sort on a 2^16 sample; then,
+                labelled separately ("threaded"), the same build with its
+                sort on up to 16 threads, once the GPU legs are done
   batched       the mixed 1024-column workload (lengths log-uniform in
                 [2^12, 2^24], cardinality log-uniform in [2, 2^16], half
                 uniform half Zipf(1.0), seed 7), resident in HBM: one batched
@@ -174,6 +176,30 @@ def finish_cpu_baseline(proc, core, gold):
            "bubble_vs_stable": r["bubble_vs_stable"]}
     if gold is not None:
         res["matches_gpu_digest"] = full["index"] == gold
+    return res
+
+
+def threaded_cpu_baseline(n, gold):
+    """The oracle build of the same column with its sort on up to 16 threads
+    (slices sorted, then merged by rank ranges: the serial build's output),
+    the other stages serial; run after the GPU legs, on the cores this
+    process may use."""
+    import indexing_amd as ia
+    from indexing_amd.digest import index_digest
+    from oracle import oracle
+    threads = max(1, min(16, len(os.sched_getaffinity(0))))
+    values = ia.generate(n, "uniform", seed=1, lo=0, hi=65535)
+    t0 = time.perf_counter()
+    r = oracle.build(values, timed=True, threads=threads)
+    dt = time.perf_counter() - t0
+    res = {"value": round(n / dt / 1e9, 5), "unit": "Gvalues/s", "cores": threads,
+           "kind": "port",
+           "sample": f"the whole 2^{n.bit_length() - 1}-value column, oracle build with the "
+                     f"sort on {threads} threads, {dt:.3f} s, text parsing excluded",
+           "stage_s": {k: round(v, 4) for k, v in r["stage_s"].items()}}
+    if gold is not None:
+        res["matches_gpu_digest"] = index_digest(r["keys"], r["offsets"], r["lengths"],
+                                                 r["words"]) == gold
     return res
 
 
@@ -731,6 +757,9 @@ def main():
         progress("waiting for the cpu baseline")
         result["cpu_baseline"] = finish_cpu_baseline(cpu[1], cpu[0],
                                                      grec["index"] if grec else None)
+        progress("threaded cpu baseline")
+        result["cpu_baseline"]["threaded"] = threaded_cpu_baseline(
+            args.n, grec["index"] if grec else None)
     print(json.dumps(result))
     if parity["equal"] is False:
         print("bench.py: the timed output differs from the oracle's", file=sys.stderr)

@@ -47,6 +47,9 @@ def lib():
             "wah_oracle_build_timed": (ctypes.c_int, [vp, sz, u32, ctypes.c_int, vp, vp, vp,
                                                       vp, ctypes.POINTER(Counts),
                                                       ctypes.POINTER(ctypes.c_double)]),
+            "wah_oracle_build_threads": (ctypes.c_int, [vp, sz, u32, ctypes.c_int, vp, vp, vp,
+                                                        vp, ctypes.POINTER(Counts),
+                                                        ctypes.POINTER(ctypes.c_double)]),
             "wah_oracle_decoded_bitmap": (sz, [vp, sz, ctypes.c_char_p, sz]),
         }
         for name, (res, args) in sig.items():
@@ -67,9 +70,10 @@ def _u32(a):
 STAGES = ("sort", "chunks", "merge", "fills", "fuse", "column_length")
 
 
-def build(values, rid_base=0, sort="packed", timed=False):
+def build(values, rid_base=0, sort="packed", timed=False, threads=1):
     """Whole pipeline of src/cpu.cpp:313-328 -> dict of numpy arrays + counts
-    (with timed=True also "stage_s": seconds per stage, STAGES order)."""
+    (with timed=True also "stage_s": seconds per stage, STAGES order).
+    threads > 1 sorts on that many threads (packed order; CPU baseline)."""
     v = _u32(values)
     n = v.size
     cap = max(n, 1)
@@ -78,8 +82,12 @@ def build(values, rid_base=0, sort="packed", timed=False):
     c = Counts()
     mode = SORT_BUBBLE if sort == "bubble" else SORT_PACKED
     st = (ctypes.c_double * len(STAGES))()
-    rc = lib().wah_oracle_build_timed(_p(v), n, rid_base, mode, _p(keys), _p(offs), _p(lens),
-                                      _p(words), ctypes.byref(c), st)
+    if threads > 1 and sort != "bubble":
+        rc = lib().wah_oracle_build_threads(_p(v), n, rid_base, threads, _p(keys), _p(offs),
+                                            _p(lens), _p(words), ctypes.byref(c), st)
+    else:
+        rc = lib().wah_oracle_build_timed(_p(v), n, rid_base, mode, _p(keys), _p(offs),
+                                          _p(lens), _p(words), ctypes.byref(c), st)
     if rc != 0:
         raise MemoryError("oracle build failed")
     K, L = int(c.keycnt), int(c.length)

@@ -16,6 +16,7 @@
  */
 #include "wah_oracle.h"
 
+#include <pthread.h>
 #include <stdlib.h>
 #include <string.h>
 #include <time.h>
@@ -75,6 +76,104 @@ static int sort_packed(uint32_t* input, uint32_t* rids, size_t n) {
   return 0;
 }
 
+/* ---- the same order on several threads (CPU baseline only) -------------
+ * Not a restatement of the reference (which is serial): the packed words are
+ * cut into T slices sorted with msort_u64 on T threads; the output is then
+ * cut into T ranges at ranks r_t = t n / T, and thread t merges the slices'
+ * parts of range t.  The words are distinct, so the word of rank r is the
+ * smallest v with count(words <= v) > r (a binary search over the 64-bit
+ * values), and each slice's part below it is the count of its words < v. */
+typedef struct {
+  uint64_t* w;           /* slices, each sorted */
+  uint64_t* out;         /* merged output (in the sort jobs: tmp space) */
+  size_t n, lo, hi;      /* sort job: slice [lo, hi) */
+  int threads, t;        /* merge job: range t of threads */
+  const size_t* bounds;  /* slice bounds, threads + 1 */
+} par_job;
+
+static size_t count_below(const uint64_t* a, size_t n, uint64_t v, int inclusive) {
+  size_t lo = 0, hi = n;  /* first index whose word is > v (inclusive) or >= v */
+  while (lo < hi) {
+    size_t mid = lo + (hi - lo) / 2;
+    if (inclusive ? a[mid] <= v : a[mid] < v) lo = mid + 1;
+    else hi = mid;
+  }
+  return lo;
+}
+
+/* cut[i] = words of slice i below the word of rank r (r = n: all). */
+static void rank_cuts(const uint64_t* w, const size_t* b, int k, size_t n, size_t r,
+                      size_t* cut) {
+  if (r >= n) {
+    for (int i = 0; i < k; ++i) cut[i] = b[i + 1] - b[i];
+    return;
+  }
+  uint64_t lo = 0, hi = UINT64_MAX;  /* smallest v with count(<= v) > r */
+  while (lo < hi) {
+    uint64_t mid = lo + (hi - lo) / 2;
+    size_t c = 0;
+    for (int i = 0; i < k; ++i) c += count_below(w + b[i], b[i + 1] - b[i], mid, 1);
+    if (c > r) hi = mid;
+    else lo = mid + 1;
+  }
+  for (int i = 0; i < k; ++i) cut[i] = count_below(w + b[i], b[i + 1] - b[i], lo, 0);
+}
+
+static void* par_sort_slice(void* p) {
+  par_job* j = (par_job*)p;
+  msort_u64(j->w + j->lo, j->out + j->lo, j->hi - j->lo);
+  return NULL;
+}
+
+static void* par_merge_range(void* p) {
+  par_job* j = (par_job*)p;
+  const int k = j->threads;
+  size_t c0[64], c1[64];
+  rank_cuts(j->w, j->bounds, k, j->n, j->n * (size_t)j->t / (size_t)k, c0);
+  rank_cuts(j->w, j->bounds, k, j->n, j->n * (size_t)(j->t + 1) / (size_t)k, c1);
+  size_t o = j->n * (size_t)j->t / (size_t)k;
+  size_t pos[64], end[64];
+  for (int i = 0; i < k; ++i) {
+    pos[i] = j->bounds[i] + c0[i];
+    end[i] = j->bounds[i] + c1[i];
+  }
+  for (;;) {
+    int best = -1;
+    for (int i = 0; i < k; ++i)
+      if (pos[i] < end[i] && (best < 0 || j->w[pos[i]] < j->w[pos[best]])) best = i;
+    if (best < 0) break;
+    j->out[o++] = j->w[pos[best]++];
+  }
+  return NULL;
+}
+
+static int sort_packed_threads(uint32_t* input, uint32_t* rids, size_t n, int threads) {
+  if (threads > 64) threads = 64;
+  if (threads < 2 || n < (size_t)threads * 1024) return sort_packed(input, rids, n);
+  uint64_t* w = (uint64_t*)malloc(n * sizeof(uint64_t));
+  uint64_t* o = (uint64_t*)malloc(n * sizeof(uint64_t));
+  if (!w || !o) { free(w); free(o); return -1; }
+  for (size_t i = 0; i < n; ++i) w[i] = ((uint64_t)input[i] << 32) | rids[i];
+  size_t b[65];
+  for (int t = 0; t <= threads; ++t) b[t] = n * (size_t)t / (size_t)threads;
+  pthread_t th[64];
+  par_job jobs[64];
+  for (int t = 0; t < threads; ++t) {
+    jobs[t] = (par_job){w, o, n, b[t], b[t + 1], threads, t, b};
+    pthread_create(&th[t], NULL, par_sort_slice, &jobs[t]);
+  }
+  for (int t = 0; t < threads; ++t) pthread_join(th[t], NULL);
+  for (int t = 0; t < threads; ++t) pthread_create(&th[t], NULL, par_merge_range, &jobs[t]);
+  for (int t = 0; t < threads; ++t) pthread_join(th[t], NULL);
+  for (size_t i = 0; i < n; ++i) {
+    input[i] = (uint32_t)(o[i] >> 32);
+    rids[i] = (uint32_t)o[i];
+  }
+  free(w);
+  free(o);
+  return 0;
+}
+
 void wah_oracle_sort_rids_by_value(uint32_t* input, uint32_t* rids, size_t n,
                                    uint32_t rid_base, int mode) {
   for (size_t i = 0; i < n; ++i) rids[i] = rid_base + (uint32_t)i;  /* iota, :182 */
@@ -90,6 +189,37 @@ void wah_oracle_produce_chunk_id_literals(const uint32_t* rids, uint32_t* chids,
     lits[i] |= 1u << 31;
     chids[i] = rids[i] / 31u;
   }
+}
+
+typedef struct {
+  const uint32_t* rids;
+  uint32_t *chids, *lits;
+  size_t lo, hi;
+} chunk_job;
+
+static void* par_chunks(void* p) {
+  chunk_job* j = (chunk_job*)p;
+  wah_oracle_produce_chunk_id_literals(j->rids + j->lo, j->chids + j->lo, j->lits + j->lo,
+                                       j->hi - j->lo);
+  return NULL;
+}
+
+/* The same on `threads` threads, each on a contiguous slice (CPU baseline). */
+static void produce_chunk_id_literals_threads(const uint32_t* rids, uint32_t* chids,
+                                              uint32_t* lits, size_t n, int threads) {
+  if (threads > 64) threads = 64;
+  if (threads < 2 || n < (size_t)threads * 1024) {
+    wah_oracle_produce_chunk_id_literals(rids, chids, lits, n);
+    return;
+  }
+  pthread_t th[64];
+  chunk_job jobs[64];
+  for (int t = 0; t < threads; ++t) {
+    jobs[t] = (chunk_job){rids, chids, lits, n * (size_t)t / (size_t)threads,
+                          n * (size_t)(t + 1) / (size_t)threads};
+    pthread_create(&th[t], NULL, par_chunks, &jobs[t]);
+  }
+  for (int t = 0; t < threads; ++t) pthread_join(th[t], NULL);
 }
 
 /* ---- merge_lit_by_val_chids: reduce_by_key OR, src/cpu.cpp:78-116 ------ */
@@ -204,10 +334,9 @@ static double now_s(void) {
 }
 
 /* ---- whole build: src/cpu.cpp:313-328 ----------------------------------- */
-int wah_oracle_build_timed(const uint32_t* values, size_t n, uint32_t rid_base,
-                           int sort_mode, uint32_t* keys, uint32_t* offsets,
-                           uint32_t* lengths, uint32_t* words, wah_oracle_counts* out,
-                           double* stage_s) {
+static int build_impl(const uint32_t* values, size_t n, uint32_t rid_base, int sort_mode,
+                      int threads, uint32_t* keys, uint32_t* offsets, uint32_t* lengths,
+                      uint32_t* words, wah_oracle_counts* out, double* stage_s) {
   double t[7] = {0};
   out->k = out->length = out->keycnt = 0;
   if (stage_s)
@@ -223,9 +352,17 @@ int wah_oracle_build_timed(const uint32_t* values, size_t n, uint32_t rid_base,
   }
   memcpy(input, values, n * sizeof(uint32_t));                   /* :313 */
   t[0] = now_s();
-  wah_oracle_sort_rids_by_value(input, rids, n, rid_base, sort_mode); /* :318 */
+  if (threads > 1) {
+    for (size_t i = 0; i < n; ++i) rids[i] = rid_base + (uint32_t)i;  /* iota, :182 */
+    if (sort_packed_threads(input, rids, n, threads)) {
+      free(input); free(rids); free(chids); free(lits);
+      return -1;
+    }
+  } else {
+    wah_oracle_sort_rids_by_value(input, rids, n, rid_base, sort_mode); /* :318 */
+  }
   t[1] = now_s();
-  wah_oracle_produce_chunk_id_literals(rids, chids, lits, n);    /* :319 */
+  produce_chunk_id_literals_threads(rids, chids, lits, n, threads); /* :319 */
   t[2] = now_s();
   size_t k = wah_oracle_merge_lit_by_val_chids(input, chids, lits, n); /* :320 */
   t[3] = now_s();
@@ -244,6 +381,22 @@ int wah_oracle_build_timed(const uint32_t* values, size_t n, uint32_t rid_base,
   out->keycnt = keycnt;
   free(input); free(rids); free(chids); free(lits);
   return 0;
+}
+
+int wah_oracle_build_timed(const uint32_t* values, size_t n, uint32_t rid_base,
+                           int sort_mode, uint32_t* keys, uint32_t* offsets,
+                           uint32_t* lengths, uint32_t* words, wah_oracle_counts* out,
+                           double* stage_s) {
+  return build_impl(values, n, rid_base, sort_mode, 1, keys, offsets, lengths, words, out,
+                    stage_s);
+}
+
+int wah_oracle_build_threads(const uint32_t* values, size_t n, uint32_t rid_base,
+                             int threads, uint32_t* keys, uint32_t* offsets,
+                             uint32_t* lengths, uint32_t* words, wah_oracle_counts* out,
+                             double* stage_s) {
+  return build_impl(values, n, rid_base, WAH_ORACLE_SORT_PACKED, threads, keys, offsets,
+                    lengths, words, out, stage_s);
 }
 
 int wah_oracle_build(const uint32_t* values, size_t n, uint32_t rid_base,

@@ -102,6 +102,15 @@ int wah_oracle_build_timed(const uint32_t* values, size_t n, uint32_t rid_base,
                            uint32_t* lengths, uint32_t* words, wah_oracle_counts* out,
                            double* stage_s);
 
+/* The same build with the sort on `threads` threads (the packed words cut
+ * into slices, sorted, then merged by rank ranges: the same order; CPU
+ * baseline only, the reference is serial).  threads <= 1 is
+ * wah_oracle_build_timed with the packed sort. */
+int wah_oracle_build_threads(const uint32_t* values, size_t n, uint32_t rid_base,
+                             int threads, uint32_t* keys, uint32_t* offsets,
+                             uint32_t* lengths, uint32_t* words, wah_oracle_counts* out,
+                             double* stage_s);
+
 /* src/cpu.cpp:47-74: '0'/'1' text of the bitmap, trailing '0's trimmed.
  * Writes at most cap bytes (no terminator) and returns the full length. */
 size_t wah_oracle_decoded_bitmap(const uint32_t* words, size_t len, char* out,

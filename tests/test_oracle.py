@@ -102,6 +102,24 @@ def test_bubble_equals_packed_sort():
                       ((np.diff(ks.astype(np.int64)) == 0) & (np.diff(rs.astype(np.int64)) > 0)))
 
 
+def test_threaded_build_equals_serial():
+    """The CPU baseline's threaded build (slices sorted on threads, merged by
+    rank ranges) gives the serial build's index: sizes below and above the
+    threading threshold, thread counts that do not divide n, few and many
+    keys, 32-bit keys and a rid_base."""
+    rng = np.random.default_rng(8)
+    cases = [(0, 4, 0), (7, 3, 0), (5000, 2, 0), (70001, 65536, 13), (300007, 300, 0),
+             (1 << 18, 1 << 32, 31 * 1000 + 7)]
+    for n, hi, base in cases:
+        v = rng.integers(0, hi, n, dtype=np.uint64).astype(np.uint32)
+        a = oracle.build(v, rid_base=base)
+        for th in (2, 3, 7, 16):
+            b = oracle.build(v, rid_base=base, threads=th)
+            assert (b["k"], b["length"], b["keycnt"]) == (a["k"], a["length"], a["keycnt"])
+            for key in ("keys", "offsets", "lengths", "words"):
+                assert np.array_equal(a[key], b[key]), (n, th, key)
+
+
 def test_empty_column():
     r = oracle.build(np.zeros(0, np.uint32))
     assert (r["k"], r["length"], r["keycnt"]) == (0, 0, 0)

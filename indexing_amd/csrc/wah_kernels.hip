@@ -137,7 +137,7 @@ __global__ __launch_bounds__(kBlock) void k_histogram(
   const Dims dims = desc->dims;
   const Column* cols = desc_cols(desc);
   for (int i = t; i < 1024; i += kBlock) h[i] = hr[i] = 0;
-  // zero look-back state used later in this build
+  // zero the look-back state the later kernels of the same index build use
   const uint64_t gthreads = (uint64_t)gridDim.x * kBlock;
   const uint64_t gt = (uint64_t)blockIdx.x * kBlock + t;
   const uint64_t s0 = (uint64_t)dims.sort_tiles * 256;

@@ -85,7 +85,9 @@ int wahgpu_find_device(const char* name);
  * library's single-pass kernels (radix passes, tail, scans, compaction,
  * decode) run one call's launch sequence at a time, in the order the calls
  * were made: each call's kernels wait (hipStreamWaitEvent) for the previous
- * call's, whatever its stream.  Their look-back waits on tiles drawn by
+ * call's, whatever its stream (a call that reads results back between its
+ * kernels, such as the staged build, orders each part between two reads
+ * separately, and no host wait holds the order).  Their look-back waits on tiles drawn by
  * workgroups placed on particular XCDs, and two such launches sharing the
  * device could each hold the CUs the other needs.  Copies and other work still
  * overlap them.  Calls made while their stream captures a graph are not

@@ -213,8 +213,12 @@ def test_mixed_shapes_in_one_batch_and_graph():
 def test_autotune_reports_a_table(ctx):
     t = ctx.autotune()
     print(f"[autotune] {t}", flush=True)
-    assert t["bucket_log2"] == [14, 17, 20, 24, 64] and t["digit_bits"] == [8] * 5
+    assert t["bucket_log2"] == [14, 17, 20, 24, 64]
+    # each bucket takes the shape, and the digit width, that its own timings
+    # favour (which one wins is the device's business, not the test's)
     for b in range(5):
-        assert t["shape"][b] in (0, 1, 2)
+        assert t["shape"][b] in range(6)
         assert all(ms > 0 for ms in t["ms"][b])
         assert t["ms"][b][t["shape"][b]] == min(t["ms"][b])
+        ms8, ms11 = t["digit_ms"][b]
+        assert t["digit_bits"][b] == (8 if b == 0 or ms8 <= ms11 else 11)

@@ -210,8 +210,14 @@ def test_mixed_shapes_in_one_batch_and_graph():
         c.close()
 
 
-def test_autotune_reports_a_table(ctx):
-    t = ctx.autotune()
+def test_autotune_reports_a_table():
+    # a context of its own: the shared one keeps its default tuning for the
+    # tests after this one
+    c = ia.Context(0)
+    try:
+        t = c.autotune()
+    finally:
+        c.close()
     print(f"[autotune] {t}", flush=True)
     assert t["bucket_log2"] == [14, 17, 20, 24, 64]
     # each bucket takes the shape, and the digit width, that its own timings

@@ -92,6 +92,19 @@ constexpr int kRunTailItems = 16;
 constexpr int kRunTailThreads = kTailTile / kRunTailItems;
 constexpr int kRunTailBlocksPerCU = 4 * 4 * 64 / kRunTailThreads;
 static_assert(kRunTailThreads * kRunTailItems == kTailTile, "run tail tiles = pair tail tiles");
+// Long single columns (at least 2^WAH_LONG_TAIL_MIN_LOG2 values) take tail
+// tiles of 16384 items, 1024 threads x 16 at one block per CU (4 waves per
+// SIMD), in both modes: 2^28 uniform values, tail 0.95 ms against 0.99 ms in
+// 8192-item tiles.  Shorter columns and batches keep the 8192-item tiles,
+// which give more CUs a tile (2^20 values: 17 us against 20 us).
+#ifndef WAH_LONG_TAIL_MIN_LOG2
+#define WAH_LONG_TAIL_MIN_LOG2 24
+#endif
+constexpr int kLongTailThreads = 1024;
+constexpr int kLongTailItems = 16;
+constexpr int kLongTailTile = kLongTailThreads * kLongTailItems;
+constexpr int kLongTailBlocksPerCU = 4 * 4 * 64 / kLongTailThreads;
+constexpr uint64_t kLongTailMin = 1ull << WAH_LONG_TAIL_MIN_LOG2;
 // waves per SIMD a tail instantiation is compiled for
 __host__ __device__ constexpr int tail_waves_per_eu(int items) {
   return items >= 16 ? 4 : kTailWavesPerEU;
@@ -251,6 +264,9 @@ struct BuildArgs {
   bool atomic_rank;
   // Single column: k_begin writes the descriptor from these (no host copy).
   bool inline_desc;
+  // The column is long: tail tiles of kLongTailTile items (one_dims counts
+  // them); tail_tiles_cap still bounds the kTailTile count.
+  bool long_tail;
   Dims one_dims;
   Column one;
 };
@@ -259,8 +275,8 @@ struct BuildArgs {
 __host__ __device__ inline uint32_t sort_tiles_of(uint64_t n, uint32_t tile = kSortTile) {
   return (uint32_t)((n + tile - 1) / tile);
 }
-__host__ __device__ inline uint32_t tail_tiles_of(uint64_t n) {
-  return (uint32_t)((n + kTailTile - 1) / kTailTile);
+__host__ __device__ inline uint32_t tail_tiles_of(uint64_t n, uint32_t tile = kTailTile) {
+  return (uint32_t)((n + tile - 1) / tile);
 }
 __host__ __device__ inline uint32_t hist_chunks_of(uint64_t n) {
   return (uint32_t)((n + kHistChunk - 1) / kHistChunk);

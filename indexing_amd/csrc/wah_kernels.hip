@@ -2083,7 +2083,24 @@ void launch_sort(const BuildArgs& a, hipStream_t s, LaunchHook* hook) {
   static_assert(kRadixShapes == 6, "a case per compiled shape");
 }
 
-void launch_tail(const BuildArgs& a, hipStream_t s, LaunchHook* hook) {
+// Both tails of a long column, on 16384-item tiles.
+static void launch_long_tail(const BuildArgs& a, hipStream_t s) {
+  const uint32_t grid =
+      cap_grid(a.tail_tiles_cap, WAH_STATIC_TILES ? a.tail_tiles_cap
+                                                  : a.max_resident / 4 * kLongTailBlocksPerCU);
+  const uint32_t lanes = WAH_TAIL_LANES ? WAH_TAIL_LANES : tile_lanes(grid);
+  hipLaunchKernelGGL((k_tail<kLongTailThreads, kLongTailItems, false>), dim3(grid),
+                     dim3(kLongTailThreads), 0, s, a.desc, a.cs, a.hdr, a.values, a.keysA,
+                     a.ridsA, a.keysB, a.ridsB, a.litsA, a.litsB, a.out_keys, a.out_offsets,
+                     a.out_words, a.tail_status, a.tile_entries, lanes, a.counts);
+  hipLaunchKernelGGL((k_tail<kLongTailThreads, kLongTailItems, true>), dim3(grid),
+                     dim3(kLongTailThreads), 0, s, a.desc, a.cs, a.hdr, a.values, a.keysA,
+                     a.ridsA, a.keysB, a.ridsB, a.litsA, a.litsB, a.out_keys, a.out_offsets,
+                     a.out_words, a.tail_status, a.tile_entries, lanes, a.counts);
+}
+
+// Both tails of other columns and of batches, on 8192-item tiles.
+static void launch_short_tail(const BuildArgs& a, hipStream_t s) {
   const uint32_t grid =
       cap_grid(a.tail_tiles_cap, WAH_STATIC_TILES ? a.tail_tiles_cap
                                                   : a.max_resident / 4 * kTailBlocksPerCU);
@@ -2102,6 +2119,13 @@ void launch_tail(const BuildArgs& a, hipStream_t s, LaunchHook* hook) {
                      dim3(kRunTailThreads), 0, s, a.desc, a.cs, a.hdr, a.values, a.keysA, a.ridsA,
                      a.keysB, a.ridsB, a.litsA, a.litsB, a.out_keys, a.out_offsets, a.out_words,
                      a.tail_status, a.tile_entries, rlanes, a.counts);
+}
+
+void launch_tail(const BuildArgs& a, hipStream_t s, LaunchHook* hook) {
+  if (a.long_tail)
+    launch_long_tail(a, s);
+  else
+    launch_short_tail(a, s);
   if (hook) hook->mark(s);
   uint64_t lg = (a.n_cap + kBlock - 1) / kBlock;
   hipLaunchKernelGGL(k_finish, dim3(cap_grid(lg > 1024 ? 1024u : (uint32_t)lg, 1024)),

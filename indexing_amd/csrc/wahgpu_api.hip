@@ -282,12 +282,14 @@ wah::DescHead* carve_desc(wahgpu_ctx* c, uint32_t cols) {
 }
 
 // A single column at offset 0: its descriptor travels in k_begin's arguments.
-// wide: it may take wide digits (slot 0, tiles from 0).
+// wide: it may take wide digits (slot 0, tiles from 0).  A long column takes
+// the long tail tiles (fewer than the caps' kTailTile count).
 void set_one(BuildArgs* a, uint64_t n, uint32_t rid_base, bool wide = false) {
   a->inline_desc = true;
+  a->long_tail = n >= wah::kLongTailMin;
   a->one_dims = wah::Dims{1u, wah::sort_tiles_of(n, wah::radix_tile(a->shape)),
-                          wah::tail_tiles_of(n), wah::hist_chunks_of(n),
-                          wide ? wide_tiles_of(n) : 0u, 0u};
+                          wah::tail_tiles_of(n, a->long_tail ? wah::kLongTailTile : wah::kTailTile),
+                          wah::hist_chunks_of(n), wide ? wide_tiles_of(n) : 0u, 0u};
   a->one = wah::Column{0, n, rid_base, 0u, 0u, 0u, 0u, wide ? 1u : 0u, 0u, 0u};
 }
 

@@ -1419,12 +1419,23 @@ __global__ __launch_bounds__(THREADS, tail_waves_per_eu(ITEMS)) void k_tail(
 #pragma unroll
       for (int i = 0; i < ITEMS; ++i) ch[i] = rid_base + j0 + (uint32_t)i;
     }
+    // ch[] held rids so far: chunk id q = rid / 31 and the literal bit
+    // rid % 31 = (rid + q) & 31 (rid + q = 32q + rid % 31).  Rows below 2^31
+    // (every column with rid_base + n <= 2^31) divide by a multiply-high,
+    // exact there (it first fails at 2369637147); the others by the
+    // compiler's full 32-bit division.
+    auto chunks = [&](auto fast_tag) {
+      constexpr bool FAST = decltype(fast_tag)::value;
 #pragma unroll
-    for (int i = 0; i < ITEMS; ++i) {  // ch[] held rids so far
-      const uint32_t r = ch[i];
-      ch[i] = r / 31u;
-      if (!RUN) b[i] = (uint32_t)i < nv ? 1u << (r - ch[i] * 31u) : 0u;
-    }
+      for (int i = 0; i < ITEMS; ++i) {
+        const uint32_t r = ch[i];
+        const uint32_t q = FAST ? __umulhi(r, 0x84210843u) >> 4 : r / 31u;
+        ch[i] = q;
+        if (!RUN) b[i] = (uint32_t)i < nv ? 1u << ((r + q) & 31u) : 0u;
+      }
+    };
+    if ((uint64_t)rid_base + cd.n <= (1ull << 31)) chunks(std::true_type{});
+    else chunks(std::false_type{});
     // predecessor of the lane's first item: lane l-1's last, or for lane 0
     // the item before the wave (one load per wave)
     uint32_t pk, pc;

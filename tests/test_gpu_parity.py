@@ -82,7 +82,10 @@ def test_staged_build_matches_oracle(ctx, name, values):
     assert set(got.times) >= {"sort", "merge", "fuse", "column_length", "total"}
 
 
-@pytest.mark.parametrize("rid_base", [1, 30, 31, 1000, 123456789])
+# rows straddling 2^31, past 2369637147 and up to 2^32 - 1 take the tail's
+# full division by 31 (below 2^31 it divides by a multiply-high)
+@pytest.mark.parametrize("rid_base", [1, 30, 31, 1000, 123456789, (1 << 31) - 2500,
+                                      3_000_000_000, (1 << 32) - 5000])
 def test_rid_base(ctx, rid_base):
     values = np.random.default_rng(rid_base).integers(0, 50, 5000).astype(np.uint32)
     got = ctx.build(values, rid_base=rid_base)

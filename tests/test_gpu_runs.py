@@ -44,6 +44,10 @@ def cases():
         ("rid30", runs_column(50000, 20, 300, 8), 30),
         ("rid31", runs_column(50000, 20, 300, 9), 31),
         ("rid_big", runs_column(70000, 50, 300, 10), 1_000_003),
+        # rows past 2^31 (the tail's full division; rows from 2369637147 on
+        # are where a multiply-high by 2^36 / 31 would be off by one)
+        ("rid_3e9", runs_column(70000, 50, 300, 11), 3_000_000_000),
+        ("rid_to_2p32", runs_column(70000, 50, 300, 12), (1 << 32) - 70000),
         ("zipf_runs", ia.generate(1 << 20, "zipf-runs", seed=3, cardinality=1 << 20, zipf_s=1.2,
                                   run_mean=4096), 0),
     ]

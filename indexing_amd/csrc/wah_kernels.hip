@@ -1466,6 +1466,12 @@ __global__ __launch_bounds__(THREADS, tail_waves_per_eu(ITEMS)) void k_tail(
       khm |= kh ? 1u << i : 0u;
       fm |= f[i] != 0u ? 1u << i : 0u;
     }
+    // From here on an item's flags are read back from these bits: the
+    // compiler would otherwise keep each item's compare masks (64-bit scalar
+    // pairs) alive to the staging and spill them to VGPR lanes.
+    stm = opaque(stm);
+    khm = opaque(khm);
+    fm = opaque(fm);
     const uint32_t nwords = (uint32_t)(__builtin_popcount(stm) + __builtin_popcount(fm));
     // lead bits: items before the lane's first stop (entry start or end of items)
     const uint32_t stops = stm | (~0u << nv & kAll);

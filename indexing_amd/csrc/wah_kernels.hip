@@ -185,42 +185,88 @@ __global__ __launch_bounds__(kBlock) void k_histogram(
         wprev[u] = lo + s0 > 0 ? src[(int64_t)s0 - 1] : 0u;
       }
       __builtin_amdgcn_sched_barrier(0);
+      // The wave's 1024 rows of the chunk (4 spans of 256).  A digit the same
+      // in all 1024 rows adds once (the high digits of small values); a digit
+      // that varies adds once per span that holds a single value (inside a
+      // run), and otherwise one LDS atomic per row.  Lane 0 adds from its own
+      // values, so no per-span scalars stay live.  The repeat histograms are
+      // added only when some row of the wave repeats its predecessor (a
+      // wave-uniform branch), so columns without repeats carry no per-row
+      // conditions on their atomics.
+      const uint32_t f0 = __builtin_amdgcn_readfirstlane(q[0].x);
+      uint32_t x = 0;    // bits in which a value of the lane differs from its span's first
+      uint32_t fx = 0;   // bits in which a span's first value differs from f0
+      uint32_t one = 0;  // bit u: span u holds a single value
+      uint32_t rm = 0;   // repeat rows of the lane: bit 4u + j
+      // row % 31 of q[u].x; spans are 1024 rows apart and 1024 = 33 * 31 + 1
+      uint32_t m = (row0 + (uint32_t)t * 4u) % 31u;
 #pragma unroll
       for (int u = 0; u < 4; ++u) {
         const uint32_t i0 = (uint32_t)(u * kBlock + t) * 4;  // index of q[u].x in src
         uint32_t prev = (uint32_t)__shfl_up((int)q[u].w, 1, kWave);
         if (lane_id() == 0) prev = wprev[u];
         const uint32_t e[4] = {q[u].x, q[u].y, q[u].z, q[u].w};
-        bool rep[4];
-        uint32_t m = (row0 + i0) % 31u;  // row % 31 of e[0]
+        uint32_t mj = m;
 #pragma unroll
         for (int j = 0; j < 4; ++j) {
           vmax = e[j] > vmax ? e[j] : vmax;
           const uint32_t pv = j == 0 ? prev : e[j - 1];
-          rep[j] = lo + i0 + j > 0 && m != 0u && e[j] == pv;
-          reps += rep[j] ? 1u : 0u;
-          m = m == 30u ? 0u : m + 1u;
+          const bool rep = lo + i0 + j > 0 && mj != 0u && e[j] == pv;
+          rm |= rep ? 1u << (4 * u + j) : 0u;
+          mj = mj == 30u ? 0u : mj + 1u;
         }
-        // Digits that are the same in all 256 rows of the wave (the high
-        // digits of small values, every digit of a run) take one add per
-        // wave; the others one LDS atomic per row.
-        const uint32_t f = __builtin_amdgcn_readfirstlane(e[0]);
-        const uint32_t vary = wave_or((e[0] ^ f) | (e[1] ^ f) | (e[2] ^ f) | (e[3] ^ f));
-        uint32_t rc = 0;
+        m = m == 30u ? 0u : m + 1u;
+        const uint32_t fu = __builtin_amdgcn_readfirstlane(e[0]);
+        const uint32_t xu = (e[0] ^ fu) | (e[1] ^ fu) | (e[2] ^ fu) | (e[3] ^ fu);
+        one |= ballot(xu != 0u) == 0 ? 1u << u : 0u;
+        x |= xu;
+        fx |= fu ^ f0;
+      }
+      reps += (uint32_t)__builtin_popcount(rm);
+      // a bit varies over the 1024 rows iff it varies inside a span or
+      // between the spans' first values
+      const uint32_t vary = wave_or(x) | fx;
 #pragma unroll
-        for (int j = 0; j < 4; ++j) rc += (uint32_t)__popcll(ballot(rep[j]));
+      for (int p = 0; p < 4; ++p) {
+        if (((vary >> (8 * p)) & 255u) == 0u) {
+          if (lane_id() == 0) atomicAdd(&h[p * 256 + ((q[0].x >> (8 * p)) & 255u)], 16u * kWave);
+          continue;
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+          const uint32_t e[4] = {q[u].x, q[u].y, q[u].z, q[u].w};
+          if ((one >> u) & 1u) {
+            if (lane_id() == 0) atomicAdd(&h[p * 256 + ((e[0] >> (8 * p)) & 255u)], 4u * kWave);
+          } else {
+#pragma unroll
+            for (int j = 0; j < 4; ++j) atomicAdd(&h[p * 256 + ((e[j] >> (8 * p)) & 255u)], 1u);
+          }
+        }
+      }
+      if (ballot(rm != 0u)) {
+        uint32_t rcu[4], rc = 0;  // repeats of each span and of the wave
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+          rcu[u] = 0;
+#pragma unroll
+          for (int j = 0; j < 4; ++j) rcu[u] += (uint32_t)__popcll(ballot((rm >> (4 * u + j)) & 1u));
+          rc += rcu[u];
+        }
 #pragma unroll
         for (int p = 0; p < 4; ++p) {
           if (((vary >> (8 * p)) & 255u) == 0u) {
-            if (lane_id() == 0) {
-              atomicAdd(&h[p * 256 + ((f >> (8 * p)) & 255u)], 4u * kWave);
-              if (rc) atomicAdd(&hr[p * 256 + ((f >> (8 * p)) & 255u)], rc);
-            }
-          } else {
+            if (lane_id() == 0) atomicAdd(&hr[p * 256 + ((q[0].x >> (8 * p)) & 255u)], rc);
+            continue;
+          }
 #pragma unroll
-            for (int j = 0; j < 4; ++j) {
-              atomicAdd(&h[p * 256 + ((e[j] >> (8 * p)) & 255u)], 1u);
-              if (rc && rep[j]) atomicAdd(&hr[p * 256 + ((e[j] >> (8 * p)) & 255u)], 1u);
+          for (int u = 0; u < 4; ++u) {
+            const uint32_t e[4] = {q[u].x, q[u].y, q[u].z, q[u].w};
+            if ((one >> u) & 1u) {
+              if (lane_id() == 0 && rcu[u]) atomicAdd(&hr[p * 256 + ((e[0] >> (8 * p)) & 255u)], rcu[u]);
+            } else {
+#pragma unroll
+              for (int j = 0; j < 4; ++j)
+                if ((rm >> (4 * u + j)) & 1u) atomicAdd(&hr[p * 256 + ((e[j] >> (8 * p)) & 255u)], 1u);
             }
           }
         }

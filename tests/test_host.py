@@ -229,3 +229,30 @@ def test_text_writer_roundtrip_and_matches_generate_cli(tmp_path):
     assert open(p, "rb").read() == open(q, "rb").read()
     ia.write_values_text(p, v[:5], sep=",", threads=1)
     assert open(p, "rb").read() == (",".join(str(x) for x in v[:5]) + "\n").encode()
+
+
+def test_tail_chunk_division_identities():
+    """The tail's chunk arithmetic (wah_kernels.hip, k_tail).  With
+    m = 0x84210843 = (2^36 + 29) / 31 and rid = 31q + k (k <= 30),
+    rid * m / 2^36 = q + (k + 29 rid / 2^36) / 31, so mulhi(rid, m) >> 4 is
+    rid // 31 whenever 29 rid < 2^36, i.e. for every rid up to 2369637128
+    (all rids below 2^31); past that it fails first where k = 30, at
+    2369637147.  Checked here at both ends of [0, 2^31), on a random sample
+    and around the failure point; and (rid + q) & 31 == rid % 31 (rid + q =
+    32q + k) on the same rids and the top of the 32-bit range."""
+    magic = np.uint64(0x84210843)
+    assert int(magic) * 31 == (1 << 36) + 29
+    rng = np.random.default_rng(31)
+    r = np.concatenate([np.arange(0, 1 << 22, dtype=np.uint64),
+                        np.arange((1 << 31) - (1 << 22), 1 << 31, dtype=np.uint64),
+                        rng.integers(0, 1 << 31, 1 << 22, dtype=np.uint64)])
+    q = (r * magic) >> np.uint64(36)
+    assert np.array_equal(q, r // np.uint64(31))
+    assert np.array_equal((r + q) & np.uint64(31), r % np.uint64(31))
+    r = np.arange(2_369_637_100, 2_369_637_200, dtype=np.uint64)
+    bad = r[((r * magic) >> np.uint64(36)) != r // np.uint64(31)]
+    assert bad[0] == 2_369_637_147 and 2_369_637_147 % 31 == 30
+    assert 29 * 2_369_637_128 < (1 << 36) <= 29 * 2_369_637_129
+    top = np.arange((1 << 32) - (1 << 22), 1 << 32, dtype=np.uint64)
+    q = top // np.uint64(31)
+    assert np.array_equal((top + q) & np.uint64(31), top % np.uint64(31))

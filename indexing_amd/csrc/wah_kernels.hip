@@ -511,16 +511,20 @@ __global__ __launch_bounds__(kBlock, 8) void k_runcut(
         pv = (uint32_t)__builtin_amdgcn_update_dpp((int)p0, (int)v[R - 1], 0x138, 0xF, 0xF, false);
       }
       const uint32_t m0 = (cd.rid_base + j0) % 31u;  // row % 31 of the lane's first row
-      uint32_t sm = 0;  // run starts
+      // Run starts, without branches: a row starts a run when it differs
+      // from its predecessor, begins a chunk (row % 31 == 0: at most one of
+      // the lane's R < 31 rows, the (31 - m0) % 31-th) or is the column's
+      // first row; rows past the column's end start nothing.
+      uint32_t sm;
       {
-        uint32_t m = m0;
+        const uint32_t ci = m0 == 0u ? 0u : 31u - m0;
+        sm = (ci < (uint32_t)R ? 1u << ci : 0u) | (j0 == 0u ? 1u : 0u);
 #pragma unroll
         for (int i = 0; i < R; ++i) {
           const uint32_t prev = i ? v[i - 1] : pv;
-          const bool st = (uint32_t)i < nv && (j0 + i == 0 || m == 0u || v[i] != prev);
-          sm |= st ? 1u << i : 0u;
-          m = m == 30u ? 0u : m + 1u;
+          sm |= (v[i] != prev ? 1u : 0u) << i;
         }
+        sm &= nv >= (uint32_t)R ? kAll : (1u << nv) - 1u;
       }
       const uint32_t stops = sm | (~0u << nv & kAll);
       const uint32_t lead = stops ? (uint32_t)__builtin_ctz(stops) : (uint32_t)R;
@@ -541,7 +545,7 @@ __global__ __launch_bounds__(kBlock, 8) void k_runcut(
         const uint32_t x = valid ? vp[g] : 0u;
         const uint32_t last = readlane(v[R - 1], kWave - 1);
         const uint32_t px = (uint32_t)__builtin_amdgcn_update_dpp((int)last, (int)x, 0x138, 0xF, 0xF, false);
-        const bool st = !valid || (cd.rid_base + g) % 31u == 0u || x != px;
+        const bool st = !valid | ((cd.rid_base + g) % 31u == 0u) | (x != px);  // no branches
         const uint64_t sb = ballot(st);  // lanes 32.. always stop
         if (l == 0) s_wlead[kWaves] = (uint32_t)__builtin_ctzll(sb);
       }

@@ -125,7 +125,8 @@ __device__ __forceinline__ void hist_flush(uint32_t* h, uint32_t* hr, uint32_t* 
   __syncthreads();
 }
 
-__global__ __launch_bounds__(kBlock) void k_histogram(
+// 8 waves per SIMD: the grid is sized for 8 resident blocks per CU
+__global__ __launch_bounds__(kBlock, 8) void k_histogram(
     const uint32_t* __restrict__ values, const DescHead* __restrict__ desc,
     ColState* __restrict__ cs, uint32_t* __restrict__ status0,
     uint64_t* __restrict__ tail_status, uint32_t* __restrict__ tile_entries,
@@ -226,47 +227,59 @@ __global__ __launch_bounds__(kBlock) void k_histogram(
       // a bit varies over the 1024 rows iff it varies inside a span or
       // between the spans' first values
       const uint32_t vary = wave_or(x) | fx;
-#pragma unroll
-      for (int p = 0; p < 4; ++p) {
-        if (((vary >> (8 * p)) & 255u) == 0u) {
-          if (lane_id() == 0) atomicAdd(&h[p * 256 + ((q[0].x >> (8 * p)) & 255u)], 16u * kWave);
-          continue;
+      if (vary == 0u) {
+        // every row of the chunk holds f0 (inside a run): lane p adds the
+        // wave's rows, and its repeat rows, to digit p's bin of f0
+        const uint32_t rc = readlane(wave_incl_add((uint32_t)__builtin_popcount(rm)), kWave - 1);
+        const uint32_t lp = lane_id();
+        if (lp < 4u) {
+          const uint32_t bin = lp * 256u + ((f0 >> (8u * lp)) & 255u);
+          atomicAdd(&h[bin], 16u * kWave);
+          if (rc) atomicAdd(&hr[bin], rc);
         }
-#pragma unroll
-        for (int u = 0; u < 4; ++u) {
-          const uint32_t e[4] = {q[u].x, q[u].y, q[u].z, q[u].w};
-          if ((one >> u) & 1u) {
-            if (lane_id() == 0) atomicAdd(&h[p * 256 + ((e[0] >> (8 * p)) & 255u)], 4u * kWave);
-          } else {
-#pragma unroll
-            for (int j = 0; j < 4; ++j) atomicAdd(&h[p * 256 + ((e[j] >> (8 * p)) & 255u)], 1u);
-          }
-        }
-      }
-      if (ballot(rm != 0u)) {
-        uint32_t rcu[4], rc = 0;  // repeats of each span and of the wave
-#pragma unroll
-        for (int u = 0; u < 4; ++u) {
-          rcu[u] = 0;
-#pragma unroll
-          for (int j = 0; j < 4; ++j) rcu[u] += (uint32_t)__popcll(ballot((rm >> (4 * u + j)) & 1u));
-          rc += rcu[u];
-        }
+      } else {
 #pragma unroll
         for (int p = 0; p < 4; ++p) {
           if (((vary >> (8 * p)) & 255u) == 0u) {
-            if (lane_id() == 0) atomicAdd(&hr[p * 256 + ((q[0].x >> (8 * p)) & 255u)], rc);
+            if (lane_id() == 0) atomicAdd(&h[p * 256 + ((q[0].x >> (8 * p)) & 255u)], 16u * kWave);
             continue;
           }
 #pragma unroll
           for (int u = 0; u < 4; ++u) {
             const uint32_t e[4] = {q[u].x, q[u].y, q[u].z, q[u].w};
             if ((one >> u) & 1u) {
-              if (lane_id() == 0 && rcu[u]) atomicAdd(&hr[p * 256 + ((e[0] >> (8 * p)) & 255u)], rcu[u]);
+              if (lane_id() == 0) atomicAdd(&h[p * 256 + ((e[0] >> (8 * p)) & 255u)], 4u * kWave);
             } else {
 #pragma unroll
-              for (int j = 0; j < 4; ++j)
-                if ((rm >> (4 * u + j)) & 1u) atomicAdd(&hr[p * 256 + ((e[j] >> (8 * p)) & 255u)], 1u);
+              for (int j = 0; j < 4; ++j) atomicAdd(&h[p * 256 + ((e[j] >> (8 * p)) & 255u)], 1u);
+            }
+          }
+        }
+        if (ballot(rm != 0u)) {
+          uint32_t rcu[4], rc = 0;  // repeats of each span and of the wave
+#pragma unroll
+          for (int u = 0; u < 4; ++u) {
+            rcu[u] = 0;
+#pragma unroll
+            for (int j = 0; j < 4; ++j) rcu[u] += (uint32_t)__popcll(ballot((rm >> (4 * u + j)) & 1u));
+            rc += rcu[u];
+          }
+#pragma unroll
+          for (int p = 0; p < 4; ++p) {
+            if (((vary >> (8 * p)) & 255u) == 0u) {
+              if (lane_id() == 0) atomicAdd(&hr[p * 256 + ((q[0].x >> (8 * p)) & 255u)], rc);
+              continue;
+            }
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+              const uint32_t e[4] = {q[u].x, q[u].y, q[u].z, q[u].w};
+              if ((one >> u) & 1u) {
+                if (lane_id() == 0 && rcu[u]) atomicAdd(&hr[p * 256 + ((e[0] >> (8 * p)) & 255u)], rcu[u]);
+              } else {
+#pragma unroll
+                for (int j = 0; j < 4; ++j)
+                  if ((rm >> (4 * u + j)) & 1u) atomicAdd(&hr[p * 256 + ((e[j] >> (8 * p)) & 255u)], 1u);
+              }
             }
           }
         }

@@ -177,13 +177,14 @@ __global__ __launch_bounds__(kBlock, 8) void k_histogram(
       const uint4* v4 = reinterpret_cast<const uint4*>(src);
       const uint32_t wv = __builtin_amdgcn_readfirstlane(t / kWave);
       uint4 q[4];
-      uint32_t wprev[4];
 #pragma unroll
       for (int u = 0; u < 4; ++u) q[u] = v4[u * kBlock + t];
-#pragma unroll
-      for (int u = 0; u < 4; ++u) {
-        const uint32_t s0 = (uint32_t)(u * kBlock + wv * kWave) * 4;  // wave's first index
-        wprev[u] = lo + s0 > 0 ? src[(int64_t)s0 - 1] : 0u;
+      // lane u < 4: the value before span u of the wave (one vector load)
+      uint32_t wprev;
+      {
+        const uint32_t lu = lane_id();
+        const uint32_t s0 = (lu * kBlock + wv * kWave) * 4;  // span's first index
+        wprev = lu < 4u && lo + s0 > 0 ? src[(int64_t)s0 - 1] : 0u;
       }
       __builtin_amdgcn_sched_barrier(0);
       // The wave's 1024 rows of the chunk (4 spans of 256).  A digit the same
@@ -204,8 +205,9 @@ __global__ __launch_bounds__(kBlock, 8) void k_histogram(
 #pragma unroll
       for (int u = 0; u < 4; ++u) {
         const uint32_t i0 = (uint32_t)(u * kBlock + t) * 4;  // index of q[u].x in src
-        uint32_t prev = (uint32_t)__shfl_up((int)q[u].w, 1, kWave);
-        if (lane_id() == 0) prev = wprev[u];
+        // lane l-1's last value (DPP wave shift); lane 0 the value before the span
+        const uint32_t prev = (uint32_t)__builtin_amdgcn_update_dpp(
+            (int)readlane(wprev, u), (int)q[u].w, 0x138, 0xF, 0xF, false);
         const uint32_t e[4] = {q[u].x, q[u].y, q[u].z, q[u].w};
         uint32_t mj = m;
 #pragma unroll
@@ -300,7 +302,7 @@ __global__ __launch_bounds__(kBlock, 8) void k_histogram(
     // runs of the wave's rows, added to their radix tile (a run-mode
     // column's cut places its items from these counts); no block barrier,
     // so the waves of a block stream their chunks independently
-    reps = (uint32_t)wave_sum64(reps);
+    reps = readlane(wave_incl_add(reps), kWave - 1);  // at most 4096
     if ((t % kWave) == 0) {
       // rows this wave covered: the fast path gives wave w rows 4 x 64 x
       // {u * 256 + w * 64 ..}; the slow path row i to thread i % 256

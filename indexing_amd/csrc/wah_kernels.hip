@@ -811,11 +811,15 @@ __global__ __launch_bounds__(THREADS, sort_waves_per_eu(ITEMS)) void k_onesweep(
       uint32_t total = 0;
       if (t < 256) {
         const uint32_t goff = col->digit_offset[shift / 8][t];
+        // all the wave counts first (independent reads), then the prefixes:
+        // interleaved, each read would wait for the write before it
+        uint32_t xw[WAVES];
+#pragma unroll
+        for (int i = 0; i < WAVES; ++i) xw[i] = s_wcnt[i][t];
 #pragma unroll
         for (int i = 0; i < WAVES; ++i) {
-          const uint32_t x = s_wcnt[i][t];
           s_wcnt[i][t] = total;
-          total += x;
+          total += xw[i];
         }
         uint32_t* my = &status[(uint64_t)tile * 256 + t];
         st_agent(my, ((tile == first ? kFlagPrefix : kFlagAgg) << 30) | total);
@@ -838,8 +842,11 @@ __global__ __launch_bounds__(THREADS, sort_waves_per_eu(ITEMS)) void k_onesweep(
         s_glob[t] -= ex;  // column index = s_glob[d] + local index
         if (t == 255) s_tilen = ex + total;
         // each wave's first slot for digit t: one LDS read per key below
+        uint32_t xw[WAVES];
 #pragma unroll
-        for (int i = 0; i < WAVES; ++i) s_wcnt[i][t] += ex;
+        for (int i = 0; i < WAVES; ++i) xw[i] = s_wcnt[i][t];
+#pragma unroll
+        for (int i = 0; i < WAVES; ++i) s_wcnt[i][t] = xw[i] + ex;
       }
       __syncthreads();
       if constexpr (PACKED) {

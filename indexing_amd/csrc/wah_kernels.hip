@@ -865,17 +865,23 @@ __global__ __launch_bounds__(THREADS, sort_waves_per_eu(ITEMS)) void k_onesweep(
         const uint32_t last = (uint32_t)(n_out - 1);  // stores stay in the column
         const uint32_t rid0 = cd.rid_base + (uint32_t)base;
         bool bad = false;
+        // the staged words, then their digits' offsets, then the stores: each
+        // batch of LDS reads issues back to back (slots past tile_n read
+        // in-bounds stale words and store nothing)
+        uint32_t xs[ITEMS], gs[ITEMS];
+#pragma unroll
+        for (int j = 0; j < ITEMS; ++j) xs[j] = s_stage[(uint32_t)j * THREADS + t];
+#pragma unroll
+        for (int j = 0; j < ITEMS; ++j) gs[j] = s_glob[(xs[j] >> (16 + shift)) & 255u];
 #pragma unroll
         for (int j = 0; j < ITEMS; ++j) {
           const uint32_t q = (uint32_t)j * THREADS + t;
           if (FULL || q < tile_n) {
-            const uint32_t x = s_stage[q];
-            const uint32_t k = x >> 16;
-            const uint32_t g = s_glob[(k >> shift) & 255u] + q;
+            const uint32_t g = gs[j] + q;
             bad |= g > last;
             const uint32_t gc = g < last ? g : last;
-            dk16[gc] = (uint16_t)k;
-            dr[gc] = rid0 + (x & 0xFFFFu);
+            dk16[gc] = (uint16_t)(xs[j] >> 16);
+            dr[gc] = rid0 + (xs[j] & 0xFFFFu);
           }
         }
         if (ballot(bad) && l == 0) atomicOr(fault, 2u);  // a scatter index past the column
@@ -914,20 +920,31 @@ __global__ __launch_bounds__(THREADS, sort_waves_per_eu(ITEMS)) void k_onesweep(
       bool bad = false;
       // destination of each slot this thread stores, kept for the rids (and
       // literal bits): the digit comes from the staged key
-      uint32_t gq[ITEMS];
+      uint32_t gq[ITEMS], kq[ITEMS];
       uint16_t* dk16 = reinterpret_cast<uint16_t*>(dk);
+      // the staged keys, then their digits' offsets, then the stores: each
+      // batch of LDS reads issues back to back (slots past tile_n read
+      // in-bounds stale keys and store nothing)
+#pragma unroll
+      for (int j = 0; j < ITEMS; ++j) kq[j] = s_stage[(uint32_t)j * THREADS + t];
+#pragma unroll
+      for (int j = 0; j < ITEMS; ++j) gq[j] = s_glob[(kq[j] >> shift) & 255u];
 #pragma unroll
       for (int j = 0; j < ITEMS; ++j) {
         const uint32_t q = (uint32_t)j * THREADS + t;
-        gq[j] = last;
-        if (FULL || q < tile_n) {
-          const uint32_t k = s_stage[q];
-          const uint32_t g = s_glob[(k >> shift) & 255u] + q;
-          bad |= g > last;
-          gq[j] = g < last ? g : last;
-          if (narrow) dk16[gq[j]] = (uint16_t)k;
-          else dk[gq[j]] = k;
-        }
+        const uint32_t g = gq[j] + q;
+        const bool in = FULL || q < tile_n;
+        bad |= in && g > last;
+        gq[j] = in && g < last ? g : last;
+      }
+      if (narrow) {
+#pragma unroll
+        for (int j = 0; j < ITEMS; ++j)
+          if (FULL || (uint32_t)j * THREADS + t < tile_n) dk16[gq[j]] = (uint16_t)kq[j];
+      } else {
+#pragma unroll
+        for (int j = 0; j < ITEMS; ++j)
+          if (FULL || (uint32_t)j * THREADS + t < tile_n) dk[gq[j]] = kq[j];
       }
       __syncthreads();
 #pragma unroll
@@ -935,10 +952,10 @@ __global__ __launch_bounds__(THREADS, sort_waves_per_eu(ITEMS)) void k_onesweep(
         if (FULL || ((part >> i) & 1u)) s_stage[(slot2[i / 2] >> (16 * (i % 2))) & 0xFFFFu] = rid[i];
       __syncthreads();
 #pragma unroll
-      for (int j = 0; j < ITEMS; ++j) {
-        const uint32_t q = (uint32_t)j * THREADS + t;
-        if (FULL || q < tile_n) dr[gq[j]] = s_stage[q];
-      }
+      for (int j = 0; j < ITEMS; ++j) kq[j] = s_stage[(uint32_t)j * THREADS + t];
+#pragma unroll
+      for (int j = 0; j < ITEMS; ++j)
+        if (FULL || (uint32_t)j * THREADS + t < tile_n) dr[gq[j]] = kq[j];
       if (RUN) {
         // literal bits of each run item, moved along with its key and rid
         __syncthreads();
@@ -949,10 +966,10 @@ __global__ __launch_bounds__(THREADS, sort_waves_per_eu(ITEMS)) void k_onesweep(
         __syncthreads();
         uint32_t* dl = dst_lits + cd.off;
 #pragma unroll
-        for (int j = 0; j < ITEMS; ++j) {
-          const uint32_t q = (uint32_t)j * THREADS + t;
-          if (FULL || q < tile_n) dl[gq[j]] = s_stage[q];
-        }
+        for (int j = 0; j < ITEMS; ++j) kq[j] = s_stage[(uint32_t)j * THREADS + t];
+#pragma unroll
+        for (int j = 0; j < ITEMS; ++j)
+          if (FULL || (uint32_t)j * THREADS + t < tile_n) dl[gq[j]] = kq[j];
       }
       if (ballot(bad) && l == 0) atomicOr(fault, 2u);  // a scatter index past the column
     };

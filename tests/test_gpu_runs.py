@@ -148,3 +148,26 @@ def test_key_widths_mixed_in_one_batch(ctx):
     for i, c in enumerate(cols):
         assert counts[i].fault == 0
         assert_same(got[i], oracle.build(c), f"column {i}")
+
+
+def span_columns():
+    """Columns of blocks of equal values (value = f(row // span)): blocks
+    shorter than, equal to and longer than the histogram's 256-row spans and
+    1024-row chunk windows, aligned to them and not, with keys that vary in
+    their low digit only (the others the same in every row) or in all four.
+    The histogram counts such chunks per span, per window or per row."""
+    n = 3 * 4096 + 1234
+    rows = np.arange(n, dtype=np.uint64)
+    out = []
+    for span in (64, 255, 256, 257, 1024, 1025, 3000):
+        b = rows // np.uint64(span)
+        out.append((f"span{span}_low", (np.uint64(0x01020300) + b % np.uint64(5)).astype(np.uint32)))
+        out.append((f"span{span}_all", ((b * np.uint64(2654435761)) % np.uint64(1 << 32)).astype(np.uint32)))
+    return out
+
+
+@pytest.mark.parametrize("name,values", span_columns(), ids=[c[0] for c in span_columns()])
+def test_span_patterns_match_oracle(ctx, name, values):
+    print(f"[spans] {name} n={values.size}", flush=True)
+    got = ctx.build(values, rid_base=7)
+    assert_same(got, oracle.build(values, rid_base=7), name)
